@@ -1,0 +1,97 @@
+"""ctypes binding of libastc_mi355x.so (include/astc_mi355x.h).
+
+The HIP library is the only implementation: if it is not built, importing the
+encoder raises instead of falling back to anything else.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(HERE, "_lib")
+LIB_PATH = os.path.join(LIB_DIR, "libastc_mi355x.so")
+SYNTH_PATH = os.path.join(LIB_DIR, "libastc_synth.so")
+
+ASTC_OK = 0
+ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "out of memory", -4: "unsupported",
+          -5: "I/O error", -6: "over budget"}
+
+FMT_RGBA8, FMT_RGB8, FMT_RG8 = 0, 1, 2
+BYTES_PER_TEXEL = {FMT_RGBA8: 4, FMT_RGB8: 3, FMT_RG8: 2}
+FLAG_ALPHA, FLAG_NORMAL, FLAG_SRGB = 1, 2, 4
+
+
+class AstcError(RuntimeError):
+    """An ASTC_ERR_* return code with the library's message."""
+
+    def __init__(self, code, message):
+        super().__init__(f"{ERRORS.get(code, code)}: {message}")
+        self.code = code
+
+
+class ImageDesc(ctypes.Structure):
+    """astc_image_desc"""
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("row_pitch", ctypes.c_int32),
+                ("src_format", ctypes.c_int32), ("block_dim", ctypes.c_int32), ("flags", ctypes.c_uint32)]
+
+
+P = ctypes.c_void_p
+I32 = ctypes.c_int
+I64 = ctypes.c_int64
+
+# name -> (restype, argtypes); every function declared in include/astc_mi355x.h
+SIGNATURES = {
+    "astc_last_error": (ctypes.c_char_p, []),
+    "astc_ctx_create": (I32, [I32, ctypes.POINTER(P)]),
+    "astc_ctx_destroy": (I32, [P]),
+    "astc_output_size": (I64, [ctypes.POINTER(ImageDesc)]),
+    "astc_encode": (I32, [P, ctypes.POINTER(ImageDesc), P, P, P]),
+    "astc_write_file": (I32, [ctypes.c_char_p, I32, I32, I32, I32, P, I64]),
+    "astc_read_file_header": (I32, [ctypes.c_char_p, ctypes.POINTER(I32), ctypes.POINTER(I32),
+                                    ctypes.POINTER(I32), ctypes.POINTER(I32), ctypes.POINTER(I64)]),
+    "astc_srgb_table": (I32, [P]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load the library once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build the HIP library first "
+                              "(python -c 'import __graft_entry__ as g; g.build()')")
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(rc):
+    """Raise AstcError for a negative return code, else return rc."""
+    if rc < 0:
+        msg = lib().astc_last_error()
+        raise AstcError(int(rc), msg.decode() if msg else "")
+    return rc
+
+
+_synth = None
+
+
+def synth_lib():
+    global _synth
+    if _synth is None:
+        if not os.path.exists(SYNTH_PATH):
+            raise ImportError(f"{SYNTH_PATH} is missing: build the libraries first")
+        s = ctypes.CDLL(SYNTH_PATH)
+        U64 = ctypes.c_uint64
+        for name in ("astc_synth_sinusoid_rgb8", "astc_synth_noise_rgba8", "astc_synth_normal_rg8"):
+            getattr(s, name).argtypes = [U64, I32, I32, P]
+            getattr(s, name).restype = None
+        s.astc_synth_downsample.argtypes = [P, I32, I32, I32, P, ctypes.POINTER(I32), ctypes.POINTER(I32)]
+        s.astc_synth_downsample.restype = None
+        _synth = s
+    return _synth

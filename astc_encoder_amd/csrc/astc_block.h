@@ -1,0 +1,380 @@
+// astc_block.h -- one ASTC block per lane: fetch, PCA endpoints, weights, ISE, block.
+//
+// Device code shared by the per-texture kernel and the persistent batched
+// kernel.  It computes what ASTC_Encode.hlsl:510-550 (encode_block) computes,
+// in a fixed floating-point schedule ("canonical schedule") that is reproduced
+// bit for bit by the CPU oracle:
+//   * sums over a block run left to right within each texel row and combine the
+//     row partials as (r0+r1)+(r2+r3) for 4x4 and ((r0+r1)+r2)+((r3+r4)+r5) for 6x6;
+//   * sums of products use fused multiply-add chains along each row, starting
+//     with a plain product;
+//   * dot products are fma(a3,b3, fma(a2,b2, fma(a1,b1, a0*b0)));
+//   * "x * s + t" expressions of the shader are single fused multiply-adds;
+//   * divisions by the constants N and N-1 are multiplications by the
+//     correctly rounded reciprocals (what FXC emits for a literal divisor);
+//   * normalize(v) = v * (1 / sqrtf(dot(v, v))) with correctly rounded sqrt and
+//     division; length(v) < 1e-5 is the equivalent test on dot(v, v).
+// The file must be compiled with -ffp-contract=off so that nothing else fuses.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "astc_format.h"
+
+namespace astc {
+
+enum : uint32_t { FLAG_ALPHA = 1u, FLAG_NORMAL = 2u, FLAG_SRGB = 4u };
+enum : int { FMT_RGBA8 = 0, FMT_RGB8 = 1, FMT_RG8 = 2 };
+
+// Tables staged into LDS by every workgroup.
+struct DeviceTables {
+    float srgb[256];    // sRGB byte -> float(linear) * 255.0f
+    uint8_t trit[256];  // 5-trit tuple index -> packed 8-bit code
+};
+
+struct Block128 { uint32_t w[4]; };
+
+__device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+__device__ __forceinline__ float dot4(const float a[4], const float b[4]) {
+    return fma_(a[3], b[3], fma_(a[2], b[2], fma_(a[1], b[1], a[0] * b[0])));
+}
+
+// correctly rounded 1 / sqrt(x) as two IEEE operations
+__device__ __forceinline__ float rsqrt_cr(float x) { return 1.0f / __builtin_sqrtf(x); }
+
+__device__ __forceinline__ bool is_short(float dot) { return dot < __uint_as_float(kLenThresholdBits); }
+
+__device__ __forceinline__ float clamp255(float x) { return fminf(fmaxf(x, 0.0f), 255.0f); }
+
+// Returns x unchanged but hides its value from the optimiser.  Used so that the
+// compiler recomputes texel - mean in the min/max pass instead of keeping the
+// covariance pass's differences alive across the power iteration (which would
+// double the registers held per block).
+__device__ __forceinline__ float opaque(float x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
+// Row-partial combination tree.
+template <int DIM>
+struct RowTree {
+    float acc0, acc1;  // DIM 4: (r0+r1), (r2+r3); DIM 6: ((r0+r1)+r2), ((r3+r4)+r5)
+    __device__ __forceinline__ void add_row(int y, float r) {
+        constexpr int half = DIM / 2;
+        if (y == 0) acc0 = r;
+        else if (y < half) acc0 = acc0 + r;
+        else if (y == half) acc1 = r;
+        else acc1 = acc1 + r;
+    }
+    __device__ __forceinline__ float total() const { return acc0 + acc1; }
+};
+
+// ---------------------------------------------------------------------------
+// Fetch (ASTC_Encode.hlsl:561-580 + D3D11 Load): out-of-bounds texels are zero;
+// RGB8 expands with alpha 255 and RG8 with blue 0, alpha 255; sRGB converts the
+// colour channels through the table; normal maps force blue = alpha = 255.
+// ---------------------------------------------------------------------------
+struct FetchArgs {
+    const uint8_t* src;
+    int width, height, row_pitch, fmt;
+    uint32_t flags;
+    int aligned;  // src and row_pitch are multiples of 4
+};
+
+__device__ __forceinline__ int bytes_per_texel(int fmt) { return fmt == FMT_RGBA8 ? 4 : fmt == FMT_RGB8 ? 3 : 2; }
+
+// Packed RGBA8 of texel (x, y) with bounds check (slow path).
+__device__ __forceinline__ uint32_t load_texel_guarded(const FetchArgs& a, int x, int y) {
+    if (x >= a.width || y >= a.height) return 0u;
+    const uint8_t* p = a.src + (size_t)y * a.row_pitch + (size_t)x * bytes_per_texel(a.fmt);
+    if (a.fmt == FMT_RGBA8) return uint32_t(p[0]) | uint32_t(p[1]) << 8 | uint32_t(p[2]) << 16 | uint32_t(p[3]) << 24;
+    if (a.fmt == FMT_RGB8) return uint32_t(p[0]) | uint32_t(p[1]) << 8 | uint32_t(p[2]) << 16 | 0xFF000000u;
+    return uint32_t(p[0]) | uint32_t(p[1]) << 8 | 0xFF000000u;
+}
+
+// One texel row of a fully in-bounds block from 4-byte-aligned words.
+template <int DIM>
+__device__ __forceinline__ void load_row_fast(const FetchArgs& a, int x0, int y, uint32_t out[DIM]) {
+    const int bpp = bytes_per_texel(a.fmt);
+    const size_t off = (size_t)y * a.row_pitch + (size_t)x0 * bpp;
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(a.src + (off & ~size_t(3)));
+    if (a.fmt == FMT_RGBA8) {
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) out[i] = w[i];
+        return;
+    }
+    // RGB8 / RG8: the row's bytes start `shift` bytes into the first word; the
+    // words are realigned with a funnel shift so that byte j sits at a fixed place.
+    constexpr int kWords = (DIM * 3 + 3) / 4 + 1;
+    const uint32_t shift = uint32_t(off & 3);
+    const int nwords = int((DIM * bpp + shift + 3) >> 2);
+    uint32_t raw[kWords + 1];
+#pragma unroll
+    for (int i = 0; i < kWords; ++i) raw[i] = i < nwords ? w[i] : 0u;
+    raw[kWords] = 0u;
+    uint32_t al[kWords];
+#pragma unroll
+    for (int i = 0; i < kWords; ++i) al[i] = __builtin_amdgcn_alignbyte(raw[i + 1], raw[i], shift);
+    auto byte_at = [&](int j) -> uint32_t { return (al[j >> 2] >> ((j & 3) * 8)) & 0xFFu; };
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        if (a.fmt == FMT_RGB8)
+            out[i] = byte_at(3 * i) | byte_at(3 * i + 1) << 8 | byte_at(3 * i + 2) << 16 | 0xFF000000u;
+        else
+            out[i] = byte_at(2 * i) | byte_at(2 * i + 1) << 8 | 0xFF000000u;
+    }
+}
+
+__device__ __forceinline__ void unpack_texel(uint32_t p, uint32_t flags, const float* lut, float t[4]) {
+    if (flags & FLAG_SRGB) {
+        t[0] = lut[p & 0xFFu];
+        t[1] = lut[(p >> 8) & 0xFFu];
+        t[2] = lut[(p >> 16) & 0xFFu];
+    } else {
+        t[0] = float(p & 0xFFu);
+        t[1] = float((p >> 8) & 0xFFu);
+        t[2] = float((p >> 16) & 0xFFu);
+    }
+    t[3] = float(p >> 24);
+    if (flags & FLAG_NORMAL) {
+        t[2] = 255.0f;
+        t[3] = 255.0f;
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ void fetch_block(const FetchArgs& a, const float* lut, int bx, int by,
+                                            float x[DIM * DIM][4]) {
+    const int x0 = bx * DIM, y0 = by * DIM;
+    const int bpp = bytes_per_texel(a.fmt);
+    // Fast path: the block is inside the image and the 4-byte words covering its
+    // row span end inside the row's texel bytes, so no load reaches past the image.
+    const bool fast = a.aligned && (x0 + DIM <= a.width) && (y0 + DIM <= a.height) &&
+                      (((x0 + DIM) * bpp + 3) & ~3) <= a.width * bpp;
+#pragma unroll
+    for (int r = 0; r < DIM; ++r) {
+        uint32_t row[DIM];
+        if (fast) {
+            load_row_fast<DIM>(a, x0, y0 + r, row);
+        } else {
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) row[i] = load_texel_guarded(a, x0 + i, y0 + r);
+        }
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) unpack_texel(row[i], a.flags, lut, x[r * DIM + i]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Integer stage: endpoint bytes + weight trits -> 128-bit block
+// (ASTC_Encode.hlsl:400-444, 475-508; ASTC_IntegerSequenceEncoding.hlsl:142-176, 207-276)
+// ---------------------------------------------------------------------------
+template <bool ALPHA>
+__device__ __forceinline__ Block128 pack_block(const uint32_t e0q[4], const uint32_t e1q[4], const uint32_t wq[16],
+                                               const uint8_t* trit_lds) {
+    constexpr WeightSplit split = ALPHA ? kSplitQ6 : kSplitQ12;
+    constexpr int b = split.bits;          // bits per weight below the trit
+    constexpr int group_bits = 5 * b + 8;  // one 5-value trit group
+    // weight stream, LSB first; at most 4 groups x 18 bits (the fourth only
+    // carries value 15, so it ends below bit 64)
+    uint64_t stream = 0;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        uint32_t t[5], m[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            const int idx = 5 * g + i;
+            if (idx < 16) {
+                t[i] = (split.trit_bits >> (2 * wq[idx])) & 3u;
+                m[i] = (split.low_bits >> (2 * wq[idx])) & 3u;
+            } else {
+                t[i] = 0u;
+                m[i] = 0u;
+            }
+        }
+        const uint32_t T = trit_lds[t[0] + 3 * t[1] + 9 * t[2] + 27 * t[3] + 81 * t[4]];
+        uint64_t grp = uint64_t(m[0]) | uint64_t(T & 3u) << b | uint64_t(m[1]) << (b + 2) |
+                       uint64_t((T >> 2) & 3u) << (2 * b + 2) | uint64_t(m[2]) << (2 * b + 4) |
+                       uint64_t((T >> 4) & 1u) << (3 * b + 4) | uint64_t(m[3]) << (3 * b + 5) |
+                       uint64_t((T >> 5) & 3u) << (4 * b + 5) | uint64_t(m[4]) << (4 * b + 7) |
+                       uint64_t(T >> 7) << (5 * b + 7);
+        stream |= grp << (g * group_bits);
+    }
+    // endpoints: QUANT_256 raw bytes r0 r1 g0 g1 b0 b1 [a0 a1] from block bit 17
+    const uint32_t ep_x = e0q[0] | e1q[0] << 8 | e0q[1] << 16 | e1q[1] << 24;
+    const uint32_t ep_y = ALPHA ? (e0q[2] | e1q[2] << 8 | e0q[3] << 16 | e1q[3] << 24) : (e0q[2] | e1q[2] << 8);
+    constexpr uint32_t mode = ALPHA ? kModeRGBA : kModeRGB;
+    constexpr uint32_t cem = ALPHA ? CEM_LDR_RGBA_DIRECT : CEM_LDR_RGB_DIRECT;
+    Block128 blk;
+    blk.w[0] = mode | (cem << 13) | (ep_x << 17);
+    blk.w[1] = (ep_x >> 15) | (ep_y << 17);
+    // weight stream bit k lands on block bit 127 - k
+    blk.w[2] = __builtin_bitreverse32(uint32_t(stream >> 32)) | (ep_y >> 15);
+    blk.w[3] = __builtin_bitreverse32(uint32_t(stream));
+    return blk;
+}
+
+// ---------------------------------------------------------------------------
+// Floating-point stage (ASTC_Encode.hlsl:93-168, 233-245, 256-393, 510-550)
+// ---------------------------------------------------------------------------
+template <int DIM, bool ALPHA>
+__device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], const uint8_t* trit_lds) {
+    constexpr int N = DIM * DIM;
+    constexpr float rcp_n = 1.0f / float(N);
+    constexpr float rcp_n1 = 1.0f / float(N - 1);
+
+    // mean (principal_component_analysis :142-147)
+    float mean[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        RowTree<DIM> tree;
+#pragma unroll
+        for (int y = 0; y < DIM; ++y) {
+            float r = x[y * DIM][c];
+#pragma unroll
+            for (int i = 1; i < DIM; ++i) r = r + x[y * DIM + i][c];
+            tree.add_row(y, r);
+        }
+        mean[c] = tree.total() * rcp_n;
+    }
+
+    // covariance (:149-162), upper triangle 00 01 02 03 11 12 13 22 23 33
+    float cov[10];
+    {
+        RowTree<DIM> tree[10];
+#pragma unroll
+        for (int y = 0; y < DIM; ++y) {
+            float r[10];
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) {
+                float d[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) d[c] = x[y * DIM + i][c] - mean[c];
+                int p = 0;
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b2 = a; b2 < 4; ++b2, ++p) r[p] = (i == 0) ? d[a] * d[b2] : fma_(d[a], d[b2], r[p]);
+            }
+#pragma unroll
+            for (int p = 0; p < 10; ++p) tree[p].add_row(y, r[p]);
+        }
+#pragma unroll
+        for (int p = 0; p < 10; ++p) cov[p] = tree[p].total() * rcp_n1;
+    }
+    const float m[4][4] = {{cov[0], cov[1], cov[2], cov[3]},
+                           {cov[1], cov[4], cov[5], cov[6]},
+                           {cov[2], cov[5], cov[7], cov[8]},
+                           {cov[3], cov[6], cov[8], cov[9]}};
+
+    // eigen_vector (:93-106): v = M v; if length(v) < eps return v; v = normalize(M v)
+    float v[4] = {0.26726f, 0.80178f, 0.53452f, 0.0f};
+    for (int it = 0; it < 8; ++it) {
+        float u[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) u[i] = dot4(m[i], v);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = u[i];
+        if (is_short(dot4(u, u))) break;
+        float w[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) w[i] = dot4(m[i], u);
+        const float s = rsqrt_cr(dot4(w, w));
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = w[i] * s;
+    }
+
+    // find_min_max (:108-137)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) mean[c] = opaque(mean[c]);
+    float lo = 1e31f, hi = -1e31f;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        float d[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) d[c] = x[k][c] - mean[c];
+        const float t = dot4(d, v);
+        lo = fminf(lo, t);
+        hi = fmaxf(hi, t);
+    }
+    float e0[4], e1[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        e0[c] = clamp255(fma_(v[c], lo, mean[c]));
+        e1[c] = clamp255(fma_(v[c], hi, mean[c]));
+    }
+    const float s0 = (rintf(e0[0]) + rintf(e0[1])) + rintf(e0[2]);
+    const float s1 = (rintf(e1[0]) + rintf(e1[1])) + rintf(e1[2]);
+    if (s0 > s1) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const float t = e0[c];
+            e0[c] = e1[c];
+            e1[c] = t;
+        }
+    }
+    if (!ALPHA) {
+        e0[3] = 255.0f;
+        e1[3] = 255.0f;
+    }
+
+    // encode_color (:233-245)
+    uint32_t e0q[4], e1q[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        e0q[c] = uint32_t(rintf(e0[c]));
+        e1q[c] = uint32_t(rintf(e1[c]));
+    }
+
+    // calculate_normal_weights (:316-372) + quantize_weights (:374-382)
+    constexpr uint32_t range = ALPHA ? 5u : 11u;
+    uint32_t wq[16];
+    float k[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) k[c] = e1[c] - e0[c];
+    const float kk = dot4(k, k);
+    if (is_short(kk)) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) wq[i] = 0u;
+    } else {
+        const float s = rsqrt_cr(kk);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) k[c] = k[c] * s;
+        float proj[16];
+        float minw = 1e31f, maxw = -1e31f;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            float diff[4];
+            if (DIM == 6) {
+                constexpr float tap_w[3] = {0.444f, 0.222f, 0.111f};
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    float smp = x[grid6_tap_index(i, 0)][c] * tap_w[grid6_tap_far(i, 0)];
+                    smp = fma_(x[grid6_tap_index(i, 1)][c], tap_w[grid6_tap_far(i, 1)], smp);
+                    smp = fma_(x[grid6_tap_index(i, 2)][c], tap_w[grid6_tap_far(i, 2)], smp);
+                    smp = fma_(x[grid6_tap_index(i, 3)][c], tap_w[grid6_tap_far(i, 3)], smp);
+                    diff[c] = smp - e0[c];
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) diff[c] = x[i][c] - e0[c];
+            }
+            const float w = dot4(k, diff);
+            minw = fminf(w, minw);
+            maxw = fmaxf(w, maxw);
+            proj[i] = w;
+        }
+        const float invlen = 1.0f / fmaxf(0.00001f, maxw - minw);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const float wn = (proj[i] - minw) * invlen;
+            const uint32_t q = uint32_t(rintf(wn * float(range)));
+            wq[i] = q < range ? q : range;
+        }
+    }
+    return pack_block<ALPHA>(e0q, e1q, wq, trit_lds);
+}
+
+}  // namespace astc

@@ -1,0 +1,193 @@
+// astc_capi.cpp -- extern "C" entry points of libastc_mi355x.so (see include/astc_mi355x.h).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "astc_internal.h"
+
+namespace astc {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+const char* last_error_cstr() { return g_last_error.c_str(); }
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg) {
+    set_error(msg);
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(ASTC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int check_desc(const astc_image_desc* d, ImageGeom* g) {
+    if (!d) return fail(ASTC_ERR_INVALID_ARG, "image description is NULL");
+    const int kMaxSize = (1 << 24) - 1;  // 24-bit sizes in the .astc header (astc_save.h:5-14)
+    if (d->width < 1 || d->height < 1 || d->width > kMaxSize || d->height > kMaxSize)
+        return fail(ASTC_ERR_INVALID_ARG, "width/height must be in 1..16777215");
+    if (d->block_dim != 4 && d->block_dim != 6) return fail(ASTC_ERR_INVALID_ARG, "block_dim must be 4 or 6");
+    int bpp = 0;
+    switch (d->src_format) {
+    case ASTC_FMT_RGBA8: bpp = 4; break;
+    case ASTC_FMT_RGB8: bpp = 3; break;
+    case ASTC_FMT_RG8: bpp = 2; break;
+    default: return fail(ASTC_ERR_INVALID_ARG, "unknown src_format");
+    }
+    if ((int64_t)d->row_pitch < (int64_t)d->width * bpp)
+        return fail(ASTC_ERR_INVALID_ARG, "row_pitch is smaller than width * bytes per texel");
+    if (d->flags & ~(ASTC_FLAG_ALPHA | ASTC_FLAG_NORMAL | ASTC_FLAG_SRGB))
+        return fail(ASTC_ERR_INVALID_ARG, "unknown flag bits");
+    const int xb = (d->width + d->block_dim - 1) / d->block_dim;
+    const int yb = (d->height + d->block_dim - 1) / d->block_dim;
+    const int64_t n = (int64_t)xb * yb;
+    if (n > (int64_t)0x7FFFFF00) return fail(ASTC_ERR_UNSUPPORTED, "more than 2^31-256 blocks in one image");
+    if (g) {
+        g->xblocks = xb;
+        g->yblocks = yb;
+        g->nblocks = n;
+        g->bpp = bpp;
+    }
+    return ASTC_OK;
+}
+
+void build_tables(DeviceTables* t) {
+    for (int c = 0; c < 256; ++c) {
+        const double u = c / 255.0;
+        const double lin = u <= 0.04045 ? u / 12.92 : std::pow((u + 0.055) / 1.055, 2.4);
+        t->srgb[c] = (float)lin * 255.0f;  // the texture unit's float, then the shader's x255 (:579)
+    }
+    std::memcpy(t->trit, kTritEncode.code, 256);
+}
+
+}  // namespace astc
+
+using namespace astc;
+
+extern "C" {
+
+const char* astc_last_error(void) { return last_error_cstr(); }
+
+int astc_ctx_create(int device, astc_ctx** out) {
+    if (!out) return fail(ASTC_ERR_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceCount");
+    if (device < 0 || device >= count) return fail(ASTC_ERR_INVALID_ARG, "device index out of range");
+    if ((e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
+    astc_ctx* ctx = new (std::nothrow) astc_ctx();
+    if (!ctx) return fail(ASTC_ERR_NO_MEMORY, "host allocation failed");
+    ctx->device = device;
+    build_tables(&ctx->h_tables);
+    if ((e = hipMalloc(&ctx->d_tables, sizeof(DeviceTables))) != hipSuccess) {
+        delete ctx;
+        return hip_fail(e, "hipMalloc(tables)");
+    }
+    if ((e = hipMemcpy(ctx->d_tables, &ctx->h_tables, sizeof(DeviceTables), hipMemcpyHostToDevice)) != hipSuccess) {
+        (void)hipFree(ctx->d_tables);
+        delete ctx;
+        return hip_fail(e, "hipMemcpy(tables)");
+    }
+    *out = ctx;
+    return ASTC_OK;
+}
+
+int astc_ctx_destroy(astc_ctx* ctx) {
+    if (!ctx) return ASTC_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->d_tables) (void)hipFree(ctx->d_tables);
+    delete ctx;
+    return ASTC_OK;
+}
+
+int64_t astc_output_size(const astc_image_desc* desc) {
+    ImageGeom g;
+    int rc = check_desc(desc, &g);
+    if (rc != ASTC_OK) return rc;
+    return g.nblocks * 16;
+}
+
+int astc_encode(astc_ctx* ctx, const astc_image_desc* desc, const void* src_dev, void* dst_dev, void* stream) {
+    if (!ctx) return fail(ASTC_ERR_INVALID_ARG, "ctx is NULL");
+    ImageGeom g;
+    int rc = check_desc(desc, &g);
+    if (rc != ASTC_OK) return rc;
+    if (!src_dev || !dst_dev) return fail(ASTC_ERR_INVALID_ARG, "src/dst is NULL");
+    if (reinterpret_cast<uintptr_t>(dst_dev) & 15u) return fail(ASTC_ERR_INVALID_ARG, "dst must be 16-byte aligned");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    e = launch_encode_plain(ctx->d_tables, *desc, g, src_dev, dst_dev, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "encode launch");
+    return ASTC_OK;
+}
+
+int astc_write_file(const char* path, int block_dim_x, int block_dim_y, int width, int height,
+                    const void* blocks_host, int64_t nbytes) {
+    if (!path || !blocks_host || nbytes < 0) return fail(ASTC_ERR_INVALID_ARG, "path/blocks invalid");
+    if (block_dim_x < 4 || block_dim_x > 12 || block_dim_y < 4 || block_dim_y > 12)
+        return fail(ASTC_ERR_INVALID_ARG, "block dims must be 4..12");
+    if (width < 1 || height < 1 || width > (1 << 24) - 1 || height > (1 << 24) - 1)
+        return fail(ASTC_ERR_INVALID_ARG, "width/height must be in 1..16777215");
+    const int64_t expect = 16LL * ((width + block_dim_x - 1) / block_dim_x) * ((height + block_dim_y - 1) / block_dim_y);
+    if (nbytes != expect) return fail(ASTC_ERR_INVALID_ARG, "nbytes does not match the block count");
+    uint8_t hdr[kAstcHeaderBytes];
+    for (int i = 0; i < 4; ++i) hdr[i] = uint8_t(kAstcMagic >> (8 * i));
+    hdr[4] = uint8_t(block_dim_x);
+    hdr[5] = uint8_t(block_dim_y);
+    hdr[6] = 1;
+    for (int i = 0; i < 3; ++i) {
+        hdr[7 + i] = uint8_t(width >> (8 * i));
+        hdr[10 + i] = uint8_t(height >> (8 * i));
+    }
+    hdr[13] = 1;
+    hdr[14] = 0;
+    hdr[15] = 0;
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(ASTC_ERR_IO, std::string("cannot open for writing: ") + path);
+    bool ok = std::fwrite(hdr, 1, sizeof(hdr), f) == sizeof(hdr) &&
+              std::fwrite(blocks_host, 1, (size_t)nbytes, f) == (size_t)nbytes;
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) return fail(ASTC_ERR_IO, std::string("short write: ") + path);
+    return ASTC_OK;
+}
+
+int astc_read_file_header(const char* path, int* block_dim_x, int* block_dim_y, int* width, int* height,
+                          int64_t* payload_bytes) {
+    if (!path) return fail(ASTC_ERR_INVALID_ARG, "path is NULL");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(ASTC_ERR_IO, std::string("cannot open: ") + path);
+    uint8_t h[kAstcHeaderBytes];
+    const size_t got = std::fread(h, 1, sizeof(h), f);
+    std::fseek(f, 0, SEEK_END);
+    const long size = std::ftell(f);
+    std::fclose(f);
+    if (got != sizeof(h)) return fail(ASTC_ERR_IO, "file shorter than the .astc header");
+    uint32_t magic = uint32_t(h[0]) | uint32_t(h[1]) << 8 | uint32_t(h[2]) << 16 | uint32_t(h[3]) << 24;
+    if (magic != kAstcMagic) return fail(ASTC_ERR_IO, "bad .astc magic");
+    const int bx = h[4], by = h[5];
+    const int w = h[7] | h[8] << 8 | h[9] << 16, ht = h[10] | h[11] << 8 | h[12] << 16;
+    if (bx < 4 || by < 4 || w < 1 || ht < 1) return fail(ASTC_ERR_IO, "bad .astc header fields");
+    const int64_t expect = 16LL * ((w + bx - 1) / bx) * ((ht + by - 1) / by);
+    if (size - (long)sizeof(h) != expect) return fail(ASTC_ERR_IO, "payload size does not match the header");
+    if (block_dim_x) *block_dim_x = bx;
+    if (block_dim_y) *block_dim_y = by;
+    if (width) *width = w;
+    if (height) *height = ht;
+    if (payload_bytes) *payload_bytes = expect;
+    return ASTC_OK;
+}
+
+int astc_srgb_table(float out[256]) {
+    if (!out) return fail(ASTC_ERR_INVALID_ARG, "out is NULL");
+    DeviceTables t;
+    build_tables(&t);
+    std::memcpy(out, t.srgb, sizeof(t.srgb));
+    return ASTC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,163 @@
+"""Host-side mirror of the reference's encoder interface on top of the C ABI.
+
+Reference names kept (niepp/astc_encoder):
+  encode_option   astc_encode.h:14-28    -> EncodeOption
+  encode_astc     astc_encode.h:87-194   -> encode_astc(texture, option)
+  save_astc       astc_save.h:52-76      -> save_astc(path, xdim, ydim, xsize, ysize, buffer)
+
+Device memory and streams come from PyTorch (plumbing only); the work is done
+by libastc_mi355x.so.
+"""
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._capi import (FLAG_ALPHA, FLAG_NORMAL, FLAG_SRGB, FMT_RG8, FMT_RGB8, FMT_RGBA8, ImageDesc,
+                    BYTES_PER_TEXEL, check, lib)
+
+_CHANNELS_TO_FMT = {4: FMT_RGBA8, 3: FMT_RGB8, 2: FMT_RG8}
+
+
+@dataclass
+class EncodeOption:
+    """encode_option (astc_encode.h:14-28).
+
+    The reference initialises is4x4 = true and never clears it, so its -6x6
+    flag has no effect (main.cpp:156-160, astc_encode.h:59,124).  Here is6x6
+    selects 6x6 blocks, which is what the flag documents (README.md:29).
+    """
+    is4x4: bool = True
+    is6x6: bool = False
+    is_normal_map: bool = False
+    has_alpha: bool = False
+    srgb: bool = False
+
+    @property
+    def block_dim(self):
+        return 6 if self.is6x6 else 4
+
+    def flags(self):
+        """Runtime flags for a texture loaded the way main.cpp:214 loads it:
+        the texture is sRGB only when srgb is set and the map is not a normal map."""
+        f = 0
+        if self.has_alpha:
+            f |= FLAG_ALPHA
+        if self.is_normal_map:
+            f |= FLAG_NORMAL
+        if self.srgb and not self.is_normal_map:
+            f |= FLAG_SRGB
+        return f
+
+
+def make_desc(width, height, src_format, block_dim, flags, row_pitch=None):
+    if row_pitch is None:
+        row_pitch = width * BYTES_PER_TEXEL[src_format]
+    return ImageDesc(int(width), int(height), int(row_pitch), int(src_format), int(block_dim), int(flags))
+
+
+def output_size(desc):
+    return check(lib().astc_output_size(ctypes.byref(desc)))
+
+
+def _stream_handle(stream):
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+class Encoder:
+    """An astc_ctx on one HIP device."""
+
+    def __init__(self, device=0):
+        self._ctx = ctypes.c_void_p()
+        check(lib().astc_ctx_create(int(device), ctypes.byref(self._ctx)))
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._ctx
+
+    def close(self):
+        if self._ctx:
+            lib().astc_ctx_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def encode_raw(self, desc, src_ptr, dst_ptr, stream=None):
+        """astc_encode on raw device pointers (asynchronous on `stream`)."""
+        check(lib().astc_encode(self._ctx, ctypes.byref(desc), ctypes.c_void_p(src_ptr),
+                                ctypes.c_void_p(dst_ptr), _stream_handle(stream)))
+
+    def encode(self, texture, block_dim=4, flags=0, out=None, stream=None):
+        """Encode a device texture (H, W, C) uint8 tensor, C in {4, 3, 2}.
+
+        Returns a (nblocks, 16) uint8 device tensor in row-major block order."""
+        import torch
+        if texture.dtype != torch.uint8 or texture.dim() != 3 or texture.shape[2] not in _CHANNELS_TO_FMT:
+            raise ValueError("texture must be an (H, W, 4|3|2) uint8 tensor")
+        if not texture.is_cuda:
+            raise ValueError("texture must be in device memory")
+        h, w, c = texture.shape
+        if texture.stride(2) != 1 or texture.stride(1) != c:
+            texture = texture.contiguous()
+        desc = make_desc(w, h, _CHANNELS_TO_FMT[c], block_dim, flags, row_pitch=texture.stride(0))
+        n = output_size(desc)
+        if out is None:
+            out = torch.empty((n // 16, 16), dtype=torch.uint8, device=texture.device)
+        elif out.numel() < n or not out.is_contiguous():
+            raise ValueError("out is too small or not contiguous")
+        self.encode_raw(desc, texture.data_ptr(), out.data_ptr(), stream)
+        return out
+
+
+_default = {}
+
+
+def _encoder_for(device_index):
+    enc = _default.get(device_index)
+    if enc is None:
+        enc = _default[device_index] = Encoder(device_index)
+    return enc
+
+
+def encode_astc(texture, option):
+    """encode_astc (astc_encode.h:87-194): encode an RGBA8 device texture with an
+    EncodeOption.  The texture's rows are in memory order (row 0 first), as the
+    reference's D3D11 texture after the loader's flip.  Returns the blocks as a
+    device tensor; the caller owns it (the reference returns a new ID3D11Buffer)."""
+    dev = texture.device.index if texture.device.index is not None else 0
+    return _encoder_for(dev).encode(texture, option.block_dim, option.flags())
+
+
+def save_astc(astc_path, xdim, ydim, xsize, ysize, buffer):
+    """save_astc (astc_save.h:52-76): 16-byte header + blocks.  Raises on I/O errors."""
+    try:
+        import torch
+        if isinstance(buffer, torch.Tensor):
+            buffer = buffer.cpu().numpy()
+    except ImportError:
+        pass
+    data = np.ascontiguousarray(np.asarray(buffer, dtype=np.uint8))
+    check(lib().astc_write_file(str(astc_path).encode(), int(xdim), int(ydim), int(xsize), int(ysize),
+                                data.ctypes.data_as(ctypes.c_void_p), data.nbytes))
+
+
+def read_astc_header(path):
+    bx, by, w, h = (ctypes.c_int() for _ in range(4))
+    n = ctypes.c_int64()
+    check(lib().astc_read_file_header(str(path).encode(), ctypes.byref(bx), ctypes.byref(by),
+                                      ctypes.byref(w), ctypes.byref(h), ctypes.byref(n)))
+    return bx.value, by.value, w.value, h.value, n.value
+
+
+def srgb_table():
+    out = np.zeros(256, np.float32)
+    check(lib().astc_srgb_table(out.ctypes.data_as(ctypes.c_void_p)))
+    return out

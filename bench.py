@@ -1,0 +1,196 @@
+"""Benchmark of the MI355X ASTC encoder; prints one JSON line.
+
+Default workload (one step = one plain per-texture launch over a resident
+texture): 4096x4096 RGBA8 sRGB, ASTC 6x6 with alpha, seeded value-noise content
+with rectangles and a cut-out alpha (466,489 blocks).  The line carries:
+  value           Mtexels/s over real texels, inputs already in HBM
+  roofline        algorithmic HBM bytes per launch / average kernel duration
+                  (HIP events on the launch stream) against 8 TB/s; `traffic`
+                  is the measured HBM bytes per launch from rocprofv3 counters
+                  (profiles/hbm_traffic_encode_plain_6x6.txt) when present
+  roofline_valu   the same launch priced in canonical FP32 operations against
+                  MI355X's 157.3 TFLOP/s vector peak
+  cpu_baseline    the CPU oracle (scalar restatement of the reference encoder,
+                  OpenMP over blocks) on this host's cores, on the same texture
+  extras          the other plain workloads (and the batched / serving forms
+                  when built)
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mtexels/s 4x4/6x6, batched vs plain at ≤0.05 dB PSNR Δ; p50/p99 req latency"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+# Canonical FP32 operation counts per block (FMA = 2; add/sub/mul/min/max/round/
+# sqrt/div = 1; loads, conversions and integer packing = 0), per mode:
+# mean C(N+1), covariance N(C + C(C+1)) + C(C+1)/2, power iteration
+# 8(2C(2C-1) + 2(2C-1) + 2 + C), min/max N(3C+1) + 8C + 11, endpoints 2C,
+# axis 6C, projections N(3C+1) (4x4) or G(10C+1) (6x6), renormalise 3 + 2G,
+# quantise 2G, with C active channels, N texels and G = 16 grid points.
+FLOPS_PER_BLOCK = {"rgb4x4": 1103, "rgba6x6": 2896, "normal4x4": 675, "rgba4x4": 1628, "rgb6x6": 1999}
+
+
+def workloads():
+    import astc_encoder_amd as astc
+    from astc_encoder_amd import synth
+    return {
+        "noise_rgba8_srgb_6x6": dict(name="noise RGBA8 sRGB 4096x4096 ASTC 6x6", gen=synth.noise_rgba8, w=4096,
+                                     h=4096, dim=6, flags=astc.FLAG_ALPHA | astc.FLAG_SRGB, bpp=4,
+                                     flops_key="rgba6x6"),
+        "sinusoid_rgb8_4x4": dict(name="sinusoid RGB8 1024x1024 ASTC 4x4", gen=synth.sinusoid_rgb8, w=1024, h=1024,
+                                  dim=4, flags=0, bpp=3, flops_key="rgb4x4"),
+        "normal_rg8_4x4": dict(name="normal RG8 2048x2048 ASTC 4x4", gen=synth.normal_rg8, w=2048, h=2048, dim=4,
+                               flags=astc.FLAG_NORMAL, bpp=2, flops_key="normal4x4"),
+    }
+
+
+def read_traffic(path):
+    """HBM bytes per launch measured by rocprofv3 counters (see the file)."""
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"^hbm_bytes_per_launch:\s*([0-9.eE+]+)", text, re.M)
+    return float(m.group(1)) if m else None
+
+
+def time_plain(enc, wl, steps, warmup):
+    import ctypes
+
+    import torch
+    import astc_encoder_amd as astc
+    img = wl["gen"](0, wl["w"], wl["h"])
+    src = torch.from_numpy(img).cuda()
+    out = enc.encode(src, block_dim=wl["dim"], flags=wl["flags"])
+    stream = torch.cuda.current_stream()
+    # one launch per step through the C ABI with everything precomputed
+    desc = astc.make_desc(wl["w"], wl["h"], {4: astc.FMT_RGBA8, 3: astc.FMT_RGB8, 2: astc.FMT_RG8}[img.shape[2]],
+                          wl["dim"], wl["flags"])
+    fn, ctx, dref = astc.lib().astc_encode, enc.handle, ctypes.byref(desc)
+    sp, dp, st = ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream.cuda_stream)
+
+    def launch():
+        if fn(ctx, dref, sp, dp, st) != 0:
+            raise astc.AstcError(-2, astc.lib().astc_last_error().decode())
+
+    for _ in range(warmup):
+        launch()
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        starts[i].record(stream)
+        launch()
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
+    texels = wl["w"] * wl["h"]
+    nblocks = out.shape[0]
+    return dict(img=img, blocks=out.cpu().numpy(), wall_s=wall, kernel_ms=kernel_ms, texels=texels,
+                nblocks=nblocks, mtexels_s=texels * steps / wall / 1e6)
+
+
+def cpu_baseline(wl, img, gpu_blocks, budget_s=20.0):
+    """The oracle (OpenMP over blocks, all cores of this host) on the same texture:
+    one warm-up, then the median of up to 5 timed runs within the budget."""
+    from tests import oracle_ref as orc
+    cores = len(os.sched_getaffinity(0))
+    t = time.perf_counter()
+    ref = orc.oracle_encode(img, wl["dim"], wl["flags"], threads=cores)
+    first = time.perf_counter() - t
+    times = []
+    for _ in range(5):
+        if sum(times) + first > budget_s:
+            break
+        t = time.perf_counter()
+        orc.oracle_encode(img, wl["dim"], wl["flags"], threads=cores)
+        times.append(time.perf_counter() - t)
+    med = float(np.median(times)) if times else first
+    match = float(np.mean(np.all(ref == gpu_blocks, axis=1)))
+    return dict(value=round(wl["w"] * wl["h"] / med / 1e6, 2), unit="Mtexels/s", cores=cores, kind="port",
+                sample=f"whole texture ({wl['name']}), median of {max(1, len(times))} runs after 1 warm-up",
+                gpu_block_match=match)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extras", action="store_true", help="only the headline workload")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (textures are independent and "
+              "would be sharded one process per GPU with no collective)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import astc_encoder_amd as astc
+
+    enc = astc.Encoder(0)
+    wls = workloads()
+    head = wls["noise_rgba8_srgb_6x6"]
+    r = time_plain(enc, head, args.steps, args.warmup)
+    algo_bytes = r["texels"] * head["bpp"] + r["nblocks"] * 16
+    achieved_gbs = algo_bytes / (r["kernel_ms"] * 1e-3) / 1e9
+    flops = FLOPS_PER_BLOCK[head["flops_key"]] * r["nblocks"]
+    achieved_tf = flops / (r["kernel_ms"] * 1e-3) / 1e12
+    traffic = read_traffic(os.path.join(ROOT, "profiles", "hbm_traffic_encode_plain_6x6.txt"))
+
+    line = {
+        "metric": METRIC,
+        "value": round(r["mtexels_s"], 1),
+        "unit": "Mtexels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["wall_s"] * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seed 0)",
+        "config": {"workload": head["name"] + ", plain per-texture launch", "width": head["w"],
+                   "height": head["h"], "block": "6x6", "flags": "alpha|srgb", "blocks": r["nblocks"],
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "encode_plain_kernel<6,true>", "kernel_ms": round(r["kernel_ms"], 4),
+                     "algorithmic_bytes": algo_bytes},
+        "roofline_valu": {"achieved": round(achieved_tf, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                          "frac": round(achieved_tf / FP32_PEAK_TFLOPS, 4), "flops_per_block":
+                          FLOPS_PER_BLOCK[head["flops_key"]]},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(head, r["img"], r["blocks"])
+    if not args.no_extras:
+        extras = {}
+        for key in ("sinusoid_rgb8_4x4", "normal_rg8_4x4"):
+            wl = wls[key]
+            e = time_plain(enc, wl, max(10, args.steps), args.warmup)
+            extras[key] = {"mtexels_s": round(e["mtexels_s"], 1), "kernel_ms": round(e["kernel_ms"], 4),
+                           "valu_tflops": round(FLOPS_PER_BLOCK[wl["flops_key"]] * e["nblocks"] /
+                                                (e["kernel_ms"] * 1e-3) / 1e12, 2)}
+        line["extras"] = extras
+    enc.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

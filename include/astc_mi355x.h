@@ -1,0 +1,101 @@
+/*
+ * astc_mi355x.h -- C ABI of the MI355X ASTC block encoder (libastc_mi355x.so).
+ *
+ * This library replaces the GPU path of niepp/astc_encoder:
+ *   encode_astc(ID3D11Device*, ID3D11DeviceContext*, ID3D11Texture2D*, const encode_option&)
+ *       (astc_encode.h:87-194) + the MainCS compute shader (ASTC_Encode.hlsl:553-582)
+ *   save_astc(path, xdim, ydim, xsize, ysize, buffer, bufsz)       (astc_save.h:52-76)
+ * with plain pointers: the caller owns device buffers and streams; the library
+ * owns only its context (device tables) and plan objects.
+ *
+ * Conventions:
+ *   - Every function returns ASTC_OK (0) or a negative ASTC_ERR_* code and never
+ *     throws; astc_last_error() gives a message for the calling thread.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the default stream).
+ *     Calls are asynchronous with respect to the host unless stated otherwise.
+ *   - Output blocks are 16 bytes each in row-major block order
+ *     (block (bx, by) at index by * ceil(W/D) + bx), exactly the contents of the
+ *     reference's output UAV (RWStructuredBuffer<uint4>, ASTC_Encode.hlsl:81).
+ */
+#ifndef ASTC_MI355X_H
+#define ASTC_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ASTC_OK 0
+#define ASTC_ERR_INVALID_ARG (-1)
+#define ASTC_ERR_HIP (-2)
+#define ASTC_ERR_NO_MEMORY (-3)
+#define ASTC_ERR_UNSUPPORTED (-4)
+#define ASTC_ERR_IO (-5)
+#define ASTC_ERR_BUDGET (-6)
+
+/* Source texel formats.  Narrower formats expand like a D3D11 texture Load:
+ * RGB8 -> (r, g, b, 255), RG8 -> (r, g, 0, 255). */
+#define ASTC_FMT_RGBA8 0
+#define ASTC_FMT_RGB8 1
+#define ASTC_FMT_RG8 2
+
+/* encode_option (astc_encode.h:14-28) as runtime flags:
+ *   ALPHA  has_alpha      -> CEM 12 (LDR RGBA direct), 6-level weights
+ *          (otherwise CEM 8, LDR RGB direct, 12-level weights, alpha ignored)
+ *   NORMAL is_normal_map  -> blue and alpha forced to 255 after the fetch
+ *   SRGB   the source is sRGB-encoded (the reference's R8G8B8A8_UNORM_SRGB
+ *          texture): colour channels are linearised on fetch.  The reference's
+ *          command line never sets it together with NORMAL (main.cpp:214). */
+#define ASTC_FLAG_ALPHA 1u
+#define ASTC_FLAG_NORMAL 2u
+#define ASTC_FLAG_SRGB 4u
+
+typedef struct astc_image_desc {
+    int32_t width;      /* texels, 1 .. 2^24-1 (the .astc header's 24-bit size) */
+    int32_t height;     /* texels, 1 .. 2^24-1 */
+    int32_t row_pitch;  /* bytes between consecutive source rows, >= width * bytes per texel */
+    int32_t src_format; /* ASTC_FMT_* */
+    int32_t block_dim;  /* 4 (ASTC 4x4) or 6 (ASTC 6x6) */
+    uint32_t flags;     /* ASTC_FLAG_* */
+} astc_image_desc;
+
+typedef struct astc_ctx astc_ctx;
+
+/* Message for the last failing call on this thread ("" if none). */
+const char* astc_last_error(void);
+
+/* Create a context on HIP device `device` (uploads the sRGB and trit tables).
+ * Replaces the reference's per-call shader compile (astc_encode.h:39-85, 92-105). */
+int astc_ctx_create(int device, astc_ctx** out);
+int astc_ctx_destroy(astc_ctx* ctx);
+
+/* Bytes of encoded output: 16 * ceil(W/D) * ceil(H/D) (astc_encode.h:127-143),
+ * or a negative error code for an invalid description. */
+int64_t astc_output_size(const astc_image_desc* desc);
+
+/* Encode one texture: replaces encode_astc (astc_encode.h:87-194) and its
+ * Dispatch of MainCS (:190).  src_dev: source texels in device memory, row 0
+ * first (the reference's texture rows, i.e. after the loader's vertical flip);
+ * dst_dev: astc_output_size(desc) bytes of device memory, 16-byte aligned.
+ * One kernel launch on `stream`. */
+int astc_encode(astc_ctx* ctx, const astc_image_desc* desc, const void* src_dev, void* dst_dev, void* stream);
+
+/* Write an .astc file: 16-byte header (magic 0x5CA1AB13, block dims, 24-bit
+ * little-endian sizes, z = 1) followed by the blocks.  Replaces save_astc
+ * (astc_save.h:52-76); unlike it, a failed open or short write is reported. */
+int astc_write_file(const char* path, int block_dim_x, int block_dim_y, int width, int height,
+                    const void* blocks_host, int64_t nbytes);
+
+/* Read an .astc file header; returns ASTC_OK and fills the fields. */
+int astc_read_file_header(const char* path, int* block_dim_x, int* block_dim_y, int* width, int* height,
+                          int64_t* payload_bytes);
+
+/* Copy the library's sRGB table (256 floats: float(linear(c/255)) * 255.0f). */
+int astc_srgb_table(float out[256]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ASTC_MI355X_H */

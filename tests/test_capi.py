@@ -1,0 +1,105 @@
+"""Host-side checks of the C ABI (no kernel launches): the library loads and
+exports every function include/*.h declares; argument validation; .astc file
+I/O; the sRGB table; the reference-shaped option semantics."""
+import ctypes
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+import astc_encoder_amd as astc
+from astc_encoder_amd import EncodeOption, _capi
+from tests import oracle_ref as orc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"^[A-Za-z_][\w\s\*]*?\b(astc_\w+)\s*\(", text, flags=re.M))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_function():
+    lib = ctypes.CDLL(_capi.LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 8
+    for n in names:
+        assert hasattr(lib, n), n
+    # and the Python binding declares a signature for each of them
+    assert set(names) <= set(_capi.SIGNATURES), set(names) - set(_capi.SIGNATURES)
+
+
+@pytest.mark.parametrize("w,h,dim,expect", [(1024, 1024, 4, 65536 * 16), (4096, 4096, 6, 683 * 683 * 16),
+                                            (1, 1, 4, 16), (7, 5, 6, 32), (16777215, 1, 4, 4194304 * 16)])
+def test_output_size(w, h, dim, expect):
+    assert astc.output_size(astc.make_desc(w, h, astc.FMT_RGBA8, dim, 0)) == expect
+
+
+@pytest.mark.parametrize("kw", [dict(width=0), dict(height=-1), dict(width=1 << 24), dict(block_dim=5),
+                                dict(src_format=7), dict(flags=8), dict(row_pitch=3)])
+def test_invalid_descriptions_are_rejected(kw):
+    args = dict(width=16, height=16, src_format=astc.FMT_RGBA8, block_dim=4, flags=0, row_pitch=64)
+    args.update(kw)
+    d = astc.ImageDesc(args["width"], args["height"], args["row_pitch"], args["src_format"], args["block_dim"],
+                       args["flags"])
+    with pytest.raises(astc.AstcError) as e:
+        astc.output_size(d)
+    assert e.value.code in (-1, -4)
+
+
+def test_encode_rejects_null_context_and_pointers():
+    d = astc.make_desc(16, 16, astc.FMT_RGBA8, 4, 0)
+    lib = astc.lib()
+    assert lib.astc_encode(None, ctypes.byref(d), ctypes.c_void_p(16), ctypes.c_void_p(16), None) == -1
+    assert b"ctx" in lib.astc_last_error()
+
+
+def test_astc_file_roundtrip(tmp_path):
+    blocks = np.arange(683 * 683 * 16, dtype=np.uint32).astype(np.uint8)
+    p = tmp_path / "t.astc"
+    astc.save_astc(p, 6, 6, 4096, 4096, blocks)
+    raw = np.fromfile(p, np.uint8)
+    assert raw[:16].tobytes() == bytes([0x13, 0xAB, 0xA1, 0x5C, 6, 6, 1, 0, 16, 0, 0, 16, 0, 1, 0, 0])
+    np.testing.assert_array_equal(raw[16:], blocks)
+    assert astc.read_astc_header(p) == (6, 6, 4096, 4096, blocks.size)
+
+
+def test_astc_file_errors(tmp_path):
+    with pytest.raises(astc.AstcError):
+        astc.save_astc(tmp_path / "no" / "such" / "dir.astc", 4, 4, 4, 4, np.zeros(16, np.uint8))
+    with pytest.raises(astc.AstcError):  # block count mismatch
+        astc.save_astc(tmp_path / "x.astc", 4, 4, 8, 8, np.zeros(16, np.uint8))
+    bad = tmp_path / "bad.astc"
+    bad.write_bytes(b"\x00" * 40)
+    with pytest.raises(astc.AstcError):
+        astc.read_astc_header(bad)
+
+
+def test_leaf_crop_fixture_header_parses(golden_dir):
+    assert astc.read_astc_header(os.path.join(golden_dir, "leaf_crop.astc")) == (4, 4, 128, 128, 16384)
+
+
+def test_srgb_table_matches_oracle():
+    np.testing.assert_array_equal(astc.srgb_table(), orc.srgb_table())
+
+
+def test_encode_option_semantics():
+    o = EncodeOption()
+    assert o.block_dim == 4 and o.flags() == 0
+    assert EncodeOption(is6x6=True).block_dim == 6
+    assert EncodeOption(has_alpha=True, srgb=True).flags() == astc.FLAG_ALPHA | astc.FLAG_SRGB
+    # main.cpp:214: a normal map is never loaded as sRGB
+    assert EncodeOption(is_normal_map=True, srgb=True).flags() == astc.FLAG_NORMAL
+
+
+def test_context_needs_a_device():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(astc.AstcError):
+        astc.Encoder(0)

@@ -1,0 +1,171 @@
+"""Known-answer tests of the oracle's floating-point stage.
+
+Each case has an answer fixed by the mathematics of ASTC_Encode.hlsl:93-168,
+316-382 rather than by re-running the same formula:
+  constant block      -> e0 = e1 = the colour, every weight 0
+  two-colour block    -> the axis is the colour difference, endpoints are the
+                         two colours, weights are 0 / max by membership
+  linear ramp         -> endpoints at the ramp's ends, weights round(range*i/15)
+  alpha-only variance -> the start vector has no alpha component, so M v = 0,
+                         the iteration returns the zero axis, e0 = e1 = mean
+  darker first        -> the swap orders endpoints by rounded RGB sum
+  RGB mode            -> endpoint alpha forced to 255, 6 endpoint bytes
+  out-of-bounds       -> padded texels fetch as 0, normal maps force b = a = 255
+plus FP64-twin and schedule-sensitivity statements on random blocks.
+"""
+import numpy as np
+import pytest
+
+from tests import oracle_ref as orc
+
+RANGE = {True: 5, False: 11}
+
+
+def decode_fields(blk, dim):
+    ok, rgba, info = orc.decode_block(blk, dim)
+    assert ok == 1
+    return rgba, info
+
+
+@pytest.mark.parametrize("dim", [4, 6])
+@pytest.mark.parametrize("alpha", [True, False])
+@pytest.mark.parametrize("sched", [orc.SCHED_CANONICAL, orc.SCHED_SOURCE])
+def test_constant_block(dim, alpha, sched):
+    col = np.array([37.0, 141.0, 200.0, 99.0 if alpha else 255.0], np.float32)
+    tex = np.tile(col, (dim * dim, 1))
+    blk, tr = orc.encode_block(tex, dim, alpha, sched)
+    assert list(tr.w_q) == [0] * 16
+    assert list(tr.ep_q[:6]) == [37, 37, 141, 141, 200, 200]
+    rgba, info = decode_fields(blk, dim)
+    assert np.all(rgba[..., :3] == col[:3].astype(np.uint8))
+    assert info.cem == (12 if alpha else 8)
+
+
+@pytest.mark.parametrize("dim", [4, 6])
+@pytest.mark.parametrize("alpha", [True, False])
+def test_two_colour_block(dim, alpha):
+    a = np.array([10.0, 20.0, 30.0, 255.0], np.float32)
+    b = np.array([210.0, 120.0, 60.0, 255.0], np.float32)
+    n = dim * dim
+    member = (np.arange(n) % 3 == 0)
+    tex = np.where(member[:, None], b, a).astype(np.float32)
+    blk, tr = orc.encode_block(tex, dim, alpha)
+    ax = np.array(tr.axis[:3])
+    d = (b - a)[:3]
+    cosang = abs(ax @ d) / (np.linalg.norm(ax) * np.linalg.norm(d))
+    assert cosang > 1 - 1e-6
+    assert list(tr.ep_q[:6]) == [10, 210, 20, 120, 30, 60]
+    if dim == 4:
+        expect = np.where(member, RANGE[alpha], 0)
+        assert list(tr.w_q) == list(expect)
+
+
+@pytest.mark.parametrize("alpha", [True, False])
+def test_linear_ramp_4x4(alpha):
+    i = np.arange(16, dtype=np.float32)
+    tex = np.stack([i * 10 + 20, i * 5 + 40, 200 - i * 8, np.full(16, 255.0, np.float32)], 1).astype(np.float32)
+    blk, tr = orc.encode_block(tex, 4, alpha)
+    assert list(tr.ep_q[:6]) == [20, 170, 40, 115, 200, 80]
+    r = RANGE[alpha]
+    # exact ties at x.5 round to even (HLSL round)
+    assert list(tr.w_q) == [int(np.rint(r * k / 15.0)) for k in range(16)]
+
+
+@pytest.mark.parametrize("dim", [4, 6])
+def test_alpha_only_variance_collapses_to_mean(dim):
+    n = dim * dim
+    tex = np.zeros((n, 4), np.float32)
+    tex[:, :3] = [90.0, 60.0, 30.0]
+    tex[:, 3] = np.where(np.arange(n) % 2 == 0, 0.0, 255.0)
+    blk, tr = orc.encode_block(tex, dim, True)
+    assert np.allclose(tr.axis, 0.0)
+    mean_a = float(np.float32(tex[:, 3].sum()) * np.float32(1.0 / n))
+    assert list(tr.ep_q) == [90, 90, 60, 60, 30, 30, int(np.rint(mean_a)), int(np.rint(mean_a))]
+    assert list(tr.w_q) == [0] * 16
+
+
+def test_darker_endpoint_first():
+    i = np.arange(16, dtype=np.float32)
+    tex = np.stack([250 - i * 10, 250 - i * 10, 250 - i * 10, np.full(16, 255.0, np.float32)], 1)
+    _, tr = orc.encode_block(tex.astype(np.float32), 4, False)
+    e0, e1 = tr.ep_q[0::2], tr.ep_q[1::2]
+    assert sum(e0[:3]) <= sum(e1[:3])
+    assert list(tr.ep_q[:2]) == [100, 250]
+
+
+def test_rgb_mode_forces_endpoint_alpha_and_ignores_it_in_weights():
+    rng = np.random.default_rng(3)
+    tex = rng.integers(0, 256, size=(16, 4)).astype(np.float32)
+    blk, tr = orc.encode_block(tex, 4, False)
+    assert tr.e0[3] == 255.0 and tr.e1[3] == 255.0
+    _, info = decode_fields(blk, 4)
+    assert info.cem == 8
+    rgba, _ = decode_fields(blk, 4)
+    assert np.all(rgba[..., 3] == 255)
+
+
+def test_fetch_out_of_bounds_and_normal_override():
+    img = np.full((5, 7, 4), 77, np.uint8)
+    t = orc.fetch_block(img, 4, 0, 1, 1)          # covers x 4..7, y 4..7 of a 7x5 image
+    inside = np.array([(x < 7 and y < 5) for y in range(4, 8) for x in range(4, 8)])
+    assert np.all(t[inside] == 77.0) and np.all(t[~inside] == 0.0)
+    tn = orc.fetch_block(img, 4, orc_flags("normal"), 1, 1)
+    assert np.all(tn[:, 2:] == 255.0) and np.all(tn[~inside, :2] == 0.0)
+    # narrower formats: RGB8 -> alpha 255, RG8 -> blue 0 alpha 255
+    t3 = orc.fetch_block(np.full((4, 4, 3), 9, np.uint8), 4, 0, 0, 0)
+    assert np.all(t3 == [9, 9, 9, 255])
+    t2 = orc.fetch_block(np.full((4, 4, 2), 9, np.uint8), 4, 0, 0, 0)
+    assert np.all(t2 == [9, 9, 0, 255])
+
+
+def orc_flags(name):
+    return {"alpha": 1, "normal": 2, "srgb": 4}[name]
+
+
+def test_srgb_table_endpoints_and_monotone():
+    lut = orc.srgb_table()
+    assert lut[0] == 0.0 and lut[255] == np.float32(255.0)
+    assert np.all(np.diff(lut) > 0)
+    c = np.arange(256) / 255.0
+    lin = np.where(c <= 0.04045, c / 12.92, ((c + 0.055) / 1.055) ** 2.4) * 255.0
+    assert np.max(np.abs(lut - lin)) < 1e-4
+    # the linear (UNORM) fetch is exact: float(c/255) * 255 == c for every byte
+    u = (np.arange(256, dtype=np.float32) / np.float32(255.0)).astype(np.float32) * np.float32(255.0)
+    assert np.array_equal(u, np.arange(256, dtype=np.float32))
+
+
+@pytest.mark.parametrize("dim,alpha", [(4, False), (4, True), (6, True), (6, False)])
+def test_fp64_twin_bounds_fp32_error(dim, alpha):
+    """The binary32 run stays close to the binary64 run of the same method."""
+    rng = np.random.default_rng(11)
+    n = dim * dim
+    worst_axis = worst_ep = 0.0
+    same = 0
+    trials = 300
+    for _ in range(trials):
+        base = rng.uniform(0, 255, 4)
+        tex = np.clip(base + rng.normal(0, 25, (n, 4)), 0, 255).round()
+        b32, t32 = orc.encode_block(tex.astype(np.float32), dim, alpha)
+        b64, t64 = orc.encode_block(tex.astype(np.float64), dim, alpha, f64=True)
+        a32, a64 = np.array(t32.axis), np.array(t64.axis)
+        if np.linalg.norm(a64) > 0:
+            worst_axis = max(worst_axis, min(np.linalg.norm(a32 - a64), np.linalg.norm(a32 + a64)))
+        worst_ep = max(worst_ep, np.max(np.abs(np.array(t32.e0) - np.array(t64.e0))),
+                       np.max(np.abs(np.array(t32.e1) - np.array(t64.e1))))
+        same += np.array_equal(b32, b64)
+    assert worst_axis < 1e-4
+    assert worst_ep < 1e-2  # in 8-bit units: far below the 0.5 that flips a rounded endpoint
+    assert same >= 0.95 * trials
+
+
+@pytest.mark.parametrize("dim,alpha", [(4, False), (6, True)])
+def test_canonical_and_source_schedules_agree_on_most_blocks(dim, alpha):
+    rng = np.random.default_rng(5)
+    n = dim * dim
+    same = 0
+    for _ in range(500):
+        tex = np.clip(rng.uniform(0, 255, 4) + rng.normal(0, 20, (n, 4)), 0, 255).round().astype(np.float32)
+        a, _ = orc.encode_block(tex, dim, alpha, orc.SCHED_CANONICAL)
+        b, _ = orc.encode_block(tex, dim, alpha, orc.SCHED_SOURCE)
+        same += np.array_equal(a, b)
+    assert same >= 0.95 * 500
