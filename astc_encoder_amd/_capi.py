@@ -34,9 +34,20 @@ class ImageDesc(ctypes.Structure):
                 ("src_format", ctypes.c_int32), ("block_dim", ctypes.c_int32), ("flags", ctypes.c_uint32)]
 
 
+class Tuning(ctypes.Structure):
+    """astc_tuning"""
+    _fields_ = [("blocks_per_lane", ctypes.c_int32), ("workgroups_per_cu", ctypes.c_int32),
+                ("prefetch", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {"blocks_per_lane": self.blocks_per_lane, "workgroups_per_cu": self.workgroups_per_cu,
+                "prefetch": self.prefetch}
+
+
 P = ctypes.c_void_p
 I32 = ctypes.c_int
 I64 = ctypes.c_int64
+PDESC = ctypes.POINTER(ImageDesc)
 
 # name -> (restype, argtypes); every function declared in include/astc_mi355x.h
 SIGNATURES = {
@@ -45,10 +56,20 @@ SIGNATURES = {
     "astc_ctx_destroy": (I32, [P]),
     "astc_output_size": (I64, [ctypes.POINTER(ImageDesc)]),
     "astc_encode": (I32, [P, ctypes.POINTER(ImageDesc), P, P, P]),
+    "astc_encode_each": (I32, [P, PDESC, ctypes.POINTER(P), ctypes.POINTER(P), I32, P]),
     "astc_write_file": (I32, [ctypes.c_char_p, I32, I32, I32, I32, P, I64]),
     "astc_read_file_header": (I32, [ctypes.c_char_p, ctypes.POINTER(I32), ctypes.POINTER(I32),
                                     ctypes.POINTER(I32), ctypes.POINTER(I32), ctypes.POINTER(I64)]),
     "astc_srgb_table": (I32, [P]),
+    "astc_mip_chain_info": (I32, [PDESC, I32, PDESC, ctypes.POINTER(I32)]),
+    "astc_downsample": (I32, [P, PDESC, P, P, P]),
+    "astc_batch_create": (I32, [P, PDESC, ctypes.POINTER(P), ctypes.POINTER(P), I32, ctypes.POINTER(P)]),
+    "astc_batch_set_tuning": (I32, [P, ctypes.POINTER(Tuning)]),
+    "astc_batch_get_tuning": (I32, [P, ctypes.POINTER(Tuning)]),
+    "astc_batch_autotune": (I32, [P, P, I32, ctypes.POINTER(Tuning)]),
+    "astc_batch_launch": (I32, [P, P]),
+    "astc_batch_info": (I32, [P, ctypes.POINTER(I64), ctypes.POINTER(I64), ctypes.POINTER(I32)]),
+    "astc_batch_destroy": (I32, [P]),
 }
 
 _lib = None

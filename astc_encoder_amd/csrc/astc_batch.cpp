@@ -1,0 +1,292 @@
+// astc_batch.cpp -- batched persistent encoding: work-list planning, hipGraph
+// capture and the autotuner (include/astc_mi355x.h, astc_batch_*).
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "astc_internal.h"
+
+using namespace astc;
+
+struct astc_batch {
+    astc_ctx* ctx = nullptr;
+    int device = 0;
+    int num_cus = 0;
+    std::vector<BatchLevel> levels;  // host copies
+    std::vector<int> modes;
+    int64_t blocks = 0, texels = 0;
+    astc_tuning tuning{4, 2, 1, 0};
+    // device work list
+    BatchLevel* d_levels = nullptr;
+    BatchSeg* d_segs = nullptr;
+    BatchItem* d_items = nullptr;
+    int* d_scalars = nullptr;  // [0] = item count, [1] = queue head
+    int nitems = 0;
+    hipStream_t capture_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace astc {
+
+// Work list for a tuning.  Modes are ordered by cost (6x6 RGBA first) so the
+// longest items start first; within a mode, images by size, large ones split
+// into chunks of 256 x blocks_per_lane blocks, images under 256 blocks packed
+// up to kMaxSegs per item (one block per thread).
+void plan_work(const std::vector<BatchLevel>& levels, const std::vector<int>& modes, int blocks_per_lane,
+               std::vector<BatchSeg>* segs, std::vector<BatchItem>* items) {
+    segs->clear();
+    items->clear();
+    const int chunk = kPlainThreads * blocks_per_lane;
+    for (int mode = MODE_6x6_RGBA; mode >= MODE_4x4_RGB; --mode) {
+        std::vector<int> idx;
+        for (int i = 0; i < (int)levels.size(); ++i)
+            if (modes[i] == mode) idx.push_back(i);
+        std::stable_sort(idx.begin(), idx.end(),
+                         [&](int a, int b) { return levels[a].nblocks > levels[b].nblocks; });
+        BatchItem pack{mode, 0, 0, 0};
+        int pack_blocks = 0;
+        auto flush = [&]() {
+            if (pack.nseg) items->push_back(pack);
+            pack = BatchItem{mode, (int)segs->size(), 0, 0};
+            pack_blocks = 0;
+        };
+        pack.seg_begin = (int)segs->size();
+        for (int li : idx) {
+            const int n = levels[li].nblocks;
+            if (n >= kPlainThreads) {
+                for (int first = 0; first < n; first += chunk) {
+                    items->push_back(BatchItem{mode, (int)segs->size(), 1, 0});
+                    segs->push_back(BatchSeg{li, first, std::min(chunk, n - first), 0});
+                }
+                pack.seg_begin = (int)segs->size();
+                continue;
+            }
+            if (pack_blocks + n > kPlainThreads || pack.nseg == kMaxSegs) {
+                // keep packs contiguous in the segment array
+                if (pack.nseg) items->push_back(pack);
+                pack = BatchItem{mode, (int)segs->size(), 0, 0};
+                pack_blocks = 0;
+            }
+            segs->push_back(BatchSeg{li, 0, n, 0});
+            ++pack.nseg;
+            pack_blocks += n;
+        }
+        flush();
+    }
+}
+
+}  // namespace astc
+
+namespace {
+
+void free_worklist(astc_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    for (void* p : {(void*)b->d_segs, (void*)b->d_items})
+        if (p) (void)hipFree(p);
+    b->d_segs = nullptr;
+    b->d_items = nullptr;
+}
+
+int build(astc_batch* b) {
+    free_worklist(b);
+    std::vector<BatchSeg> segs;
+    std::vector<BatchItem> items;
+    plan_work(b->levels, b->modes, b->tuning.blocks_per_lane, &segs, &items);
+    hipError_t e;
+    if ((e = hipMalloc(&b->d_segs, std::max<size_t>(1, segs.size()) * sizeof(BatchSeg))) != hipSuccess)
+        return hip_fail(e, "hipMalloc(segs)");
+    if ((e = hipMalloc(&b->d_items, std::max<size_t>(1, items.size()) * sizeof(BatchItem))) != hipSuccess)
+        return hip_fail(e, "hipMalloc(items)");
+    if (!segs.empty() &&
+        (e = hipMemcpy(b->d_segs, segs.data(), segs.size() * sizeof(BatchSeg), hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, "hipMemcpy(segs)");
+    if (!items.empty() && (e = hipMemcpy(b->d_items, items.data(), items.size() * sizeof(BatchItem),
+                                         hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, "hipMemcpy(items)");
+    b->nitems = (int)items.size();
+    int scal[2] = {b->nitems, 0};
+    if ((e = hipMemcpy(b->d_scalars, scal, sizeof(scal), hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, "hipMemcpy(scalars)");
+
+    BatchArgs a;
+    a.levels = b->d_levels;
+    a.segs = b->d_segs;
+    a.items = b->d_items;
+    a.nitems = b->d_scalars;
+    a.counter = b->d_scalars + 1;
+    a.tables = b->ctx->d_tables;
+    a.prefetch = b->tuning.prefetch;
+    const int grid = std::max(1, b->num_cus * b->tuning.workgroups_per_cu);
+    if ((e = hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+        return hip_fail(e, "hipStreamBeginCapture");
+    hipError_t le = launch_encode_batch(a, grid, b->capture_stream);
+    e = hipStreamEndCapture(b->capture_stream, &b->graph);
+    if (le != hipSuccess) return hip_fail(le, "batch launch capture");
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    if ((e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0)) != hipSuccess)
+        return hip_fail(e, "hipGraphInstantiate");
+    return ASTC_OK;
+}
+
+bool valid_tuning(const astc_tuning* t) {
+    const int bpl = t->blocks_per_lane;
+    return (bpl == 1 || bpl == 2 || bpl == 4 || bpl == 8) && t->workgroups_per_cu >= 1 &&
+           t->workgroups_per_cu <= 4 && (t->prefetch == 0 || t->prefetch == 1);
+}
+
+}  // namespace
+
+extern "C" {
+
+int astc_batch_create(astc_ctx* ctx, const astc_image_desc* descs, const void* const* srcs, void* const* dsts,
+                      int count, astc_batch** out) {
+    if (!out) return fail(ASTC_ERR_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    if (!ctx || !descs || !srcs || !dsts || count < 1) return fail(ASTC_ERR_INVALID_ARG, "empty or NULL batch");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    astc_batch* b = new (std::nothrow) astc_batch();
+    if (!b) return fail(ASTC_ERR_NO_MEMORY, "host allocation failed");
+    b->ctx = ctx;
+    b->device = ctx->device;
+    b->levels.resize(count);
+    b->modes.resize(count);
+    for (int i = 0; i < count; ++i) {
+        ImageGeom g;
+        int rc = check_desc(&descs[i], &g);
+        if (rc != ASTC_OK || !srcs[i] || !dsts[i] || (reinterpret_cast<uintptr_t>(dsts[i]) & 15u)) {
+            delete b;
+            if (rc == ASTC_OK) rc = fail(ASTC_ERR_INVALID_ARG, "image " + std::to_string(i) + ": bad src/dst pointer");
+            else set_error("image " + std::to_string(i) + ": " + astc_last_error());
+            return rc;
+        }
+        BatchLevel& L = b->levels[i];
+        L.f.src = static_cast<const uint8_t*>(srcs[i]);
+        L.f.width = descs[i].width;
+        L.f.height = descs[i].height;
+        L.f.row_pitch = descs[i].row_pitch;
+        L.f.fmt = descs[i].src_format;
+        L.f.flags = descs[i].flags;
+        L.f.aligned = ((reinterpret_cast<uintptr_t>(srcs[i]) & 3u) == 0) && ((descs[i].row_pitch & 3) == 0);
+        L.dst = static_cast<uint32_t*>(dsts[i]);
+        L.xblocks = g.xblocks;
+        L.nblocks = (int)g.nblocks;
+        b->modes[i] = mode_of(descs[i].block_dim, descs[i].flags);
+        b->blocks += g.nblocks;
+        b->texels += (int64_t)descs[i].width * descs[i].height;
+    }
+    int rc = ASTC_OK;
+    if ((e = hipDeviceGetAttribute(&b->num_cus, hipDeviceAttributeMultiprocessorCount, ctx->device)) != hipSuccess)
+        rc = hip_fail(e, "hipDeviceGetAttribute");
+    else if ((e = hipMalloc(&b->d_levels, count * sizeof(BatchLevel))) != hipSuccess)
+        rc = hip_fail(e, "hipMalloc(levels)");
+    else if ((e = hipMemcpy(b->d_levels, b->levels.data(), count * sizeof(BatchLevel), hipMemcpyHostToDevice)) !=
+             hipSuccess)
+        rc = hip_fail(e, "hipMemcpy(levels)");
+    else if ((e = hipMalloc(&b->d_scalars, 2 * sizeof(int))) != hipSuccess)
+        rc = hip_fail(e, "hipMalloc(scalars)");
+    else if ((e = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking)) != hipSuccess)
+        rc = hip_fail(e, "hipStreamCreate");
+    else
+        rc = build(b);
+    if (rc != ASTC_OK) {
+        astc_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return ASTC_OK;
+}
+
+int astc_batch_set_tuning(astc_batch* b, const astc_tuning* t) {
+    if (!b || !t) return fail(ASTC_ERR_INVALID_ARG, "NULL argument");
+    if (!valid_tuning(t)) return fail(ASTC_ERR_INVALID_ARG, "tuning out of range");
+    hipError_t e = hipSetDevice(b->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    b->tuning = *t;
+    b->tuning.reserved = 0;
+    return build(b);
+}
+
+int astc_batch_get_tuning(const astc_batch* b, astc_tuning* t) {
+    if (!b || !t) return fail(ASTC_ERR_INVALID_ARG, "NULL argument");
+    *t = b->tuning;
+    return ASTC_OK;
+}
+
+int astc_batch_launch(astc_batch* b, void* stream) {
+    if (!b || !b->exec) return fail(ASTC_ERR_INVALID_ARG, "batch is NULL or not built");
+    hipError_t e = hipSetDevice(b->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    if ((e = hipGraphLaunch(b->exec, static_cast<hipStream_t>(stream))) != hipSuccess)
+        return hip_fail(e, "hipGraphLaunch");
+    return ASTC_OK;
+}
+
+int astc_batch_autotune(astc_batch* b, void* stream, int reps, astc_tuning* best_out) {
+    if (!b) return fail(ASTC_ERR_INVALID_ARG, "batch is NULL");
+    if (reps < 1) reps = 1;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipEvent_t t0, t1;
+    hipError_t e;
+    if ((e = hipEventCreate(&t0)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+    if ((e = hipEventCreate(&t1)) != hipSuccess) {
+        (void)hipEventDestroy(t0);
+        return hip_fail(e, "hipEventCreate");
+    }
+    astc_tuning best = b->tuning;
+    float best_ms = 1e30f;
+    int rc = ASTC_OK;
+    for (int bpl : {1, 2, 4, 8})
+        for (int wpc : {1, 2})
+            for (int pf : {0, 1}) {
+                if (bpl == 1 && pf == 1) continue;  // nothing to prefetch within an item
+                astc_tuning t{bpl, wpc, pf, 0};
+                if ((rc = astc_batch_set_tuning(b, &t)) != ASTC_OK) break;
+                if ((rc = astc_batch_launch(b, stream)) != ASTC_OK) break;  // warm-up
+                (void)hipEventRecord(t0, s);
+                for (int r = 0; r < reps && rc == ASTC_OK; ++r) rc = astc_batch_launch(b, stream);
+                (void)hipEventRecord(t1, s);
+                if ((e = hipEventSynchronize(t1)) != hipSuccess) {
+                    rc = hip_fail(e, "hipEventSynchronize");
+                    break;
+                }
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, t0, t1);
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = t;
+                }
+            }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (rc != ASTC_OK) return rc;
+    if ((rc = astc_batch_set_tuning(b, &best)) != ASTC_OK) return rc;
+    if (best_out) *best_out = best;
+    return ASTC_OK;
+}
+
+int astc_batch_info(const astc_batch* b, int64_t* blocks, int64_t* texels, int32_t* items) {
+    if (!b) return fail(ASTC_ERR_INVALID_ARG, "batch is NULL");
+    if (blocks) *blocks = b->blocks;
+    if (texels) *texels = b->texels;
+    if (items) *items = b->nitems;
+    return ASTC_OK;
+}
+
+int astc_batch_destroy(astc_batch* b) {
+    if (!b) return ASTC_OK;
+    (void)hipSetDevice(b->device);
+    free_worklist(b);
+    if (b->d_levels) (void)hipFree(b->d_levels);
+    if (b->d_scalars) (void)hipFree(b->d_scalars);
+    if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
+    delete b;
+    return ASTC_OK;
+}
+
+}  // extern "C"

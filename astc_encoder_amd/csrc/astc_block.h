@@ -144,27 +144,32 @@ __device__ __forceinline__ void unpack_texel(uint32_t p, uint32_t flags, const f
     }
 }
 
+// Load a block's texels as packed RGBA8 words (loads only; no conversion), so
+// that a caller can issue the next block's loads before encoding this one.
 template <int DIM>
-__device__ __forceinline__ void fetch_block(const FetchArgs& a, const float* lut, int bx, int by,
-                                            float x[DIM * DIM][4]) {
+__device__ __forceinline__ void load_block(const FetchArgs& a, int bx, int by, uint32_t p[DIM * DIM]) {
     const int x0 = bx * DIM, y0 = by * DIM;
     const int bpp = bytes_per_texel(a.fmt);
     // Fast path: the block is inside the image and the 4-byte words covering its
     // row span end inside the row's texel bytes, so no load reaches past the image.
     const bool fast = a.aligned && (x0 + DIM <= a.width) && (y0 + DIM <= a.height) &&
                       (((x0 + DIM) * bpp + 3) & ~3) <= a.width * bpp;
+    if (fast) {
 #pragma unroll
-    for (int r = 0; r < DIM; ++r) {
-        uint32_t row[DIM];
-        if (fast) {
-            load_row_fast<DIM>(a, x0, y0 + r, row);
-        } else {
+        for (int r = 0; r < DIM; ++r) load_row_fast<DIM>(a, x0, y0 + r, p + r * DIM);
+    } else {
 #pragma unroll
-            for (int i = 0; i < DIM; ++i) row[i] = load_texel_guarded(a, x0 + i, y0 + r);
-        }
+        for (int r = 0; r < DIM; ++r)
 #pragma unroll
-        for (int i = 0; i < DIM; ++i) unpack_texel(row[i], a.flags, lut, x[r * DIM + i]);
+            for (int i = 0; i < DIM; ++i) p[r * DIM + i] = load_texel_guarded(a, x0 + i, y0 + r);
     }
+}
+
+template <int DIM>
+__device__ __forceinline__ void unpack_block(const uint32_t p[DIM * DIM], uint32_t flags, const float* lut,
+                                             float x[DIM * DIM][4]) {
+#pragma unroll
+    for (int k = 0; k < DIM * DIM; ++k) unpack_texel(p[k], flags, lut, x[k]);
 }
 
 // ---------------------------------------------------------------------------

@@ -1,4 +1,5 @@
 // astc_capi.cpp -- extern "C" entry points of libastc_mi355x.so (see include/astc_mi355x.h).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -126,6 +127,16 @@ int astc_encode(astc_ctx* ctx, const astc_image_desc* desc, const void* src_dev,
     return ASTC_OK;
 }
 
+int astc_encode_each(astc_ctx* ctx, const astc_image_desc* descs, const void* const* srcs, void* const* dsts,
+                     int count, void* stream) {
+    if (!descs || !srcs || !dsts || count < 0) return fail(ASTC_ERR_INVALID_ARG, "NULL arrays");
+    for (int i = 0; i < count; ++i) {
+        int rc = astc_encode(ctx, &descs[i], srcs[i], dsts[i], stream);
+        if (rc != ASTC_OK) return rc;
+    }
+    return ASTC_OK;
+}
+
 int astc_write_file(const char* path, int block_dim_x, int block_dim_y, int width, int height,
                     const void* blocks_host, int64_t nbytes) {
     if (!path || !blocks_host || nbytes < 0) return fail(ASTC_ERR_INVALID_ARG, "path/blocks invalid");
@@ -179,6 +190,46 @@ int astc_read_file_header(const char* path, int* block_dim_x, int* block_dim_y, 
     if (width) *width = w;
     if (height) *height = ht;
     if (payload_bytes) *payload_bytes = expect;
+    return ASTC_OK;
+}
+
+int astc_mip_chain_info(const astc_image_desc* base, int max_levels, astc_image_desc* levels, int* count) {
+    ImageGeom g;
+    int rc = check_desc(base, &g);
+    if (rc != ASTC_OK) return rc;
+    if (!count) return fail(ASTC_ERR_INVALID_ARG, "count is NULL");
+    int w = base->width, h = base->height, n = 0;
+    for (;;) {
+        if (levels && n < max_levels) {
+            levels[n] = *base;
+            levels[n].width = w;
+            levels[n].height = h;
+            levels[n].row_pitch = w * g.bpp;
+        }
+        ++n;
+        if (w < 8 && h < 8) break;
+        w = w >= 8 ? w >> 1 : w;
+        h = h >= 8 ? h >> 1 : h;
+    }
+    *count = n;
+    return ASTC_OK;
+}
+
+int astc_downsample(astc_ctx* ctx, const astc_image_desc* d, const void* src_dev, void* dst_dev, void* stream) {
+    if (!ctx) return fail(ASTC_ERR_INVALID_ARG, "ctx is NULL");
+    ImageGeom g;
+    int rc = check_desc(d, &g);
+    if (rc != ASTC_OK) return rc;
+    if (d->row_pitch != d->width * g.bpp) return fail(ASTC_ERR_INVALID_ARG, "mip levels must be tightly packed");
+    if (!src_dev || !dst_dev) return fail(ASTC_ERR_INVALID_ARG, "src/dst is NULL");
+    if (d->width < 8 && d->height < 8) return fail(ASTC_ERR_INVALID_ARG, "no side can halve (both below 8)");
+    const int ow = d->width >= 8 ? d->width >> 1 : d->width;
+    const int oh = d->height >= 8 ? d->height >> 1 : d->height;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    e = launch_downsample(static_cast<const uint8_t*>(src_dev), d->width, d->height, g.bpp,
+                          static_cast<uint8_t*>(dst_dev), ow, oh, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "downsample launch");
     return ASTC_OK;
 }
 

@@ -5,9 +5,13 @@
 // floating-point operations per block and hundreds of thousands of blocks per
 // texture, keeping each block in one lane avoids cross-lane reductions and
 // replicated serial work (the 16-step power iteration), so every lane issues
-// useful VALU work.  A workgroup of 256 lanes covers 256 consecutive blocks in
-// row-major block order, so the lanes of a wave read adjacent block rows of the
-// texture and write 16-byte blocks to consecutive addresses.
+// useful VALU work.  A workgroup of 256 lanes covers BPL x 256 consecutive
+// blocks in row-major block order: in step k lane t encodes block
+// base + 256 k + t, so the lanes of a wave read adjacent block rows of the
+// texture and write 16-byte blocks to consecutive addresses.  Each lane issues
+// the loads of its next block before encoding the current one, which hides the
+// HBM latency behind ~2k VALU instructions even at two waves per SIMD (the
+// 6x6 encoder holds 144 texel floats per lane).
 #include "astc_internal.h"
 
 namespace astc {
@@ -20,8 +24,6 @@ struct PlainArgs {
     const DeviceTables* tables;
 };
 
-constexpr int kPlainThreads = 256;
-
 __device__ __forceinline__ void stage_tables(const DeviceTables* t, float* lut, uint8_t* trit) {
     const int tid = threadIdx.x;
     for (int i = tid; i < 256; i += blockDim.x) {
@@ -30,24 +32,175 @@ __device__ __forceinline__ void stage_tables(const DeviceTables* t, float* lut, 
     }
 }
 
+template <int DIM>
+__device__ __forceinline__ void load_block_id(const FetchArgs& f, int xblocks, int id, uint32_t p[DIM * DIM]) {
+    const int by = id / xblocks;
+    load_block<DIM>(f, id - by * xblocks, by, p);
+}
+
+__device__ __forceinline__ void store_block(uint32_t* dst, int id, const Block128& b) {
+    reinterpret_cast<uint4*>(dst)[id] = make_uint4(b.w[0], b.w[1], b.w[2], b.w[3]);
+}
+
+// Encode blocks [first, first + count) of one image with the workgroup's
+// threads: step k, thread t takes block first + k * blockDim.x + t.  With
+// prefetch the next step's texels are loaded before the current block is
+// encoded.
 template <int DIM, bool ALPHA>
-__global__ void __launch_bounds__(kPlainThreads) encode_plain_kernel(PlainArgs a) {
+__device__ __forceinline__ void encode_chunk(const FetchArgs& f, int xblocks, uint32_t* dst, int first, int count,
+                                             bool prefetch, const float* lut, const uint8_t* trit) {
+    const int stride = blockDim.x;
+    int o = threadIdx.x;
+    if (o >= count) return;
+    uint32_t cur[DIM * DIM];
+    load_block_id<DIM>(f, xblocks, first + o, cur);
+#pragma unroll 1
+    for (;; o += stride) {
+        const bool has_next = o + stride < count;
+        uint32_t nxt[DIM * DIM];
+        if (has_next && prefetch) load_block_id<DIM>(f, xblocks, first + o + stride, nxt);
+        float x[DIM * DIM][4];
+        unpack_block<DIM>(cur, f.flags, lut, x);
+        store_block(dst, first + o, encode_block<DIM, ALPHA>(x, trit));
+        if (!has_next) break;
+        if (prefetch) {
+#pragma unroll
+            for (int i = 0; i < DIM * DIM; ++i) cur[i] = nxt[i];
+        } else {
+            load_block_id<DIM>(f, xblocks, first + o + stride, cur);
+        }
+    }
+}
+
+template <int DIM, bool ALPHA>
+__global__ void __launch_bounds__(kPlainThreads) encode_plain_kernel(PlainArgs a, int bpl) {
     __shared__ float lut[256];
     __shared__ uint8_t trit[256];
     stage_tables(a.tables, lut, trit);
     __syncthreads();
-    const int id = blockIdx.x * kPlainThreads + threadIdx.x;
-    if (id >= a.nblocks) return;
-    const int by = id / a.xblocks;
-    const int bx = id - by * a.xblocks;
-    float x[DIM * DIM][4];
-    fetch_block<DIM>(a.f, lut, bx, by, x);
-    const Block128 blk = encode_block<DIM, ALPHA>(x, trit);
-    reinterpret_cast<uint4*>(a.dst)[id] = make_uint4(blk.w[0], blk.w[1], blk.w[2], blk.w[3]);
+    const int chunk = kPlainThreads * bpl;
+    const int first = blockIdx.x * chunk;
+    const int count = min(chunk, a.nblocks - first);
+    encode_chunk<DIM, ALPHA>(a.f, a.xblocks, a.dst, first, count, true, lut, trit);
+}
+
+// ---------------------------------------------------------------------------
+// Batched persistent kernel: workgroups take work items from a device queue
+// (atomic counter) until it is empty.  An item is either a chunk of one image
+// (up to 256 x bpl blocks) or a pack of up to kMaxSegs small images (at most
+// one block per thread).  Items are uniform in mode (block size x alpha), so
+// each workgroup runs one encoder specialisation per item.
+// ---------------------------------------------------------------------------
+__global__ void reset_counter_kernel(int* counter) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
+}
+
+template <int DIM, bool ALPHA>
+__device__ __forceinline__ void process_item(const BatchArgs& a, const BatchItem& item, const float* lut,
+                                             const uint8_t* trit) {
+    if (item.nseg == 1) {
+        const BatchSeg sg = a.segs[item.seg_begin];
+        const BatchLevel& L = a.levels[sg.level];
+        encode_chunk<DIM, ALPHA>(L.f, L.xblocks, L.dst, sg.first_block, sg.count, a.prefetch != 0, lut, trit);
+        return;
+    }
+    int o = threadIdx.x;
+    for (int s = 0; s < item.nseg; ++s) {
+        const BatchSeg sg = a.segs[item.seg_begin + s];
+        if (o < sg.count) {
+            const BatchLevel& L = a.levels[sg.level];
+            uint32_t p[DIM * DIM];
+            load_block_id<DIM>(L.f, L.xblocks, sg.first_block + o, p);
+            float x[DIM * DIM][4];
+            unpack_block<DIM>(p, L.f.flags, lut, x);
+            store_block(L.dst, sg.first_block + o, encode_block<DIM, ALPHA>(x, trit));
+            return;
+        }
+        o -= sg.count;
+    }
+}
+
+__global__ void __launch_bounds__(kPlainThreads) encode_batch_kernel(BatchArgs a) {
+    __shared__ float lut[256];
+    __shared__ uint8_t trit[256];
+    __shared__ int next_item;
+    stage_tables(a.tables, lut, trit);
+    const int nitems = *a.nitems;
+    for (;;) {
+        __syncthreads();  // tables staged / everyone done with the previous item index
+        if (threadIdx.x == 0) next_item = atomicAdd(a.counter, 1);
+        __syncthreads();
+        const int it = __builtin_amdgcn_readfirstlane(next_item);
+        if (it >= nitems) break;
+        const BatchItem item = a.items[it];
+        switch (item.mode) {
+        case MODE_4x4_RGB: process_item<4, false>(a, item, lut, trit); break;
+        case MODE_4x4_RGBA: process_item<4, true>(a, item, lut, trit); break;
+        case MODE_6x6_RGB: process_item<6, false>(a, item, lut, trit); break;
+        default: process_item<6, true>(a, item, lut, trit); break;
+        }
+    }
+}
+
+hipError_t launch_encode_batch(const BatchArgs& a, int grid, hipStream_t stream) {
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, a.counter);
+    hipLaunchKernelGGL(encode_batch_kernel, dim3(unsigned(grid)), dim3(kPlainThreads), 0, stream, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Mip generation: next level of a tightly packed 8-bit image; a side of at
+// least 8 halves (rounding down) and a shorter side is kept, so 2 * out <= in
+// and every tap is inside the source.  2x2 box filter in stored space,
+// (a+b+c+d+2)>>2, or (a+b+1)>>1 along the one axis that halves.  One thread per
+// output byte.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) downsample_kernel(const uint8_t* __restrict__ src, int w, int h, int bpp,
+                                                        uint8_t* __restrict__ dst, int ow, int oh) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t total = int64_t(ow) * oh * bpp;
+    if (i >= total) return;
+    const int c = int(i % bpp);
+    const int64_t t = i / bpp;
+    const int x = int(t % ow), y = int(t / ow);
+    const bool hx = ow < w, hy = oh < h;
+    const int sx = hx ? 2 * x : x, sy = hy ? 2 * y : y;
+    auto at = [&](int xx, int yy) { return int(src[(int64_t(yy) * w + xx) * bpp + c]); };
+    int v;
+    if (hx && hy) v = (at(sx, sy) + at(sx + 1, sy) + at(sx, sy + 1) + at(sx + 1, sy + 1) + 2) >> 2;
+    else if (hx) v = (at(sx, sy) + at(sx + 1, sy) + 1) >> 1;
+    else if (hy) v = (at(sx, sy) + at(sx, sy + 1) + 1) >> 1;
+    else v = at(sx, sy);
+    dst[i] = uint8_t(v);
+}
+
+hipError_t launch_downsample(const uint8_t* src, int w, int h, int bpp, uint8_t* dst, int ow, int oh,
+                             hipStream_t stream) {
+    const int64_t total = int64_t(ow) * oh * bpp;
+    hipLaunchKernelGGL(downsample_kernel, dim3(unsigned((total + 255) / 256)), dim3(256), 0, stream, src, w, h, bpp,
+                       dst, ow, oh);
+    return hipGetLastError();
+}
+
+template <int DIM, bool ALPHA>
+static void launch_plain_t(const PlainArgs& a, int64_t nblocks, int bpl, hipStream_t stream) {
+    const dim3 grid(unsigned((nblocks + kPlainThreads * bpl - 1) / (kPlainThreads * bpl))), block(kPlainThreads);
+    hipLaunchKernelGGL((encode_plain_kernel<DIM, ALPHA>), grid, block, 0, stream, a, bpl);
+}
+
+// Blocks per lane for a plain launch: enough workgroups to give every SIMD of
+// the 256 CUs a few waves, then as many blocks per lane as that allows (up to 8)
+// so that the prefetch of the next block overlaps the current one.
+int plain_blocks_per_lane(int64_t nblocks, int dim) {
+    const int64_t waves_wanted = int64_t(256) * 4 * (dim == 6 ? 2 : 4);
+    const int64_t lanes = waves_wanted * 64;
+    int bpl = 1;
+    while (bpl < 8 && nblocks >= lanes * bpl * 2) bpl *= 2;
+    return bpl;
 }
 
 hipError_t launch_encode_plain(const DeviceTables* d_tables, const astc_image_desc& d, const ImageGeom& g,
-                               const void* src, void* dst, hipStream_t stream) {
+                               const void* src, void* dst, hipStream_t stream, int bpl_override) {
     PlainArgs a;
     a.f.src = static_cast<const uint8_t*>(src);
     a.f.width = d.width;
@@ -60,14 +213,14 @@ hipError_t launch_encode_plain(const DeviceTables* d_tables, const astc_image_de
     a.nblocks = int(g.nblocks);
     a.dst = static_cast<uint32_t*>(dst);
     a.tables = d_tables;
-    const dim3 grid(unsigned((g.nblocks + kPlainThreads - 1) / kPlainThreads)), block(kPlainThreads);
     const bool alpha = d.flags & FLAG_ALPHA;
+    const int bpl = bpl_override > 0 ? bpl_override : plain_blocks_per_lane(g.nblocks, d.block_dim);
     if (d.block_dim == 4) {
-        if (alpha) hipLaunchKernelGGL((encode_plain_kernel<4, true>), grid, block, 0, stream, a);
-        else hipLaunchKernelGGL((encode_plain_kernel<4, false>), grid, block, 0, stream, a);
+        if (alpha) launch_plain_t<4, true>(a, g.nblocks, bpl, stream);
+        else launch_plain_t<4, false>(a, g.nblocks, bpl, stream);
     } else {
-        if (alpha) hipLaunchKernelGGL((encode_plain_kernel<6, true>), grid, block, 0, stream, a);
-        else hipLaunchKernelGGL((encode_plain_kernel<6, false>), grid, block, 0, stream, a);
+        if (alpha) launch_plain_t<6, true>(a, g.nblocks, bpl, stream);
+        else launch_plain_t<6, false>(a, g.nblocks, bpl, stream);
     }
     return hipGetLastError();
 }

@@ -17,7 +17,8 @@
 //                   differences, n = normalize(-s dh/dx, -s dh/dy, 1),
 //                   RG = round((n.xy * 0.5 + 0.5) * 255)
 //   mip chain       2x2 box filter in stored space, (a+b+c+d+2)>>2, or
-//                   (a+b+1)>>1 along the one axis that still halves, down to 4x4
+//                   (a+b+1)>>1 along the one axis that still halves; a side
+//                   halves while it is at least 8 (power-of-two sides end at 4)
 #include <stdint.h>
 
 #include <algorithm>
@@ -222,9 +223,10 @@ void astc_synth_normal_rg8(uint64_t seed, int width, int height, uint8_t* out) {
 }
 
 // Next mip level of a tightly packed image with `bpp` bytes per texel.
-// Returns the new size through *nw, *nh.  dims: max(4, w/2) x max(4, h/2).
+// Returns the new size through *nw, *nh: a side of at least 8 halves (rounding
+// down), a shorter side is kept, so the filter never reads past the image.
 void astc_synth_downsample(const uint8_t* src, int w, int h, int bpp, uint8_t* dst, int* nw, int* nh) {
-    const int ow = std::max(4, w >> 1), oh = std::max(4, h >> 1);
+    const int ow = w >= 8 ? w >> 1 : w, oh = h >= 8 ? h >> 1 : h;
     const bool hx = ow < w, hy = oh < h;
 #pragma omp parallel for schedule(static)
     for (int y = 0; y < oh; ++y)

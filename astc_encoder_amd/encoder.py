@@ -14,7 +14,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from ._capi import (FLAG_ALPHA, FLAG_NORMAL, FLAG_SRGB, FMT_RG8, FMT_RGB8, FMT_RGBA8, ImageDesc,
-                    BYTES_PER_TEXEL, check, lib)
+                    BYTES_PER_TEXEL, Tuning, check, lib)
 
 _CHANNELS_TO_FMT = {4: FMT_RGBA8, 3: FMT_RGB8, 2: FMT_RG8}
 
@@ -115,6 +115,104 @@ class Encoder:
             raise ValueError("out is too small or not contiguous")
         self.encode_raw(desc, texture.data_ptr(), out.data_ptr(), stream)
         return out
+
+
+    def downsample(self, level, stream=None):
+        """Next mip level of a device (H, W, C) uint8 tensor (2x2 box filter)."""
+        import torch
+        h, w, c = level.shape
+        desc = make_desc(w, h, _CHANNELS_TO_FMT[c], 4, 0)
+        ow = w >> 1 if w >= 8 else w
+        oh = h >> 1 if h >= 8 else h
+        out = torch.empty((oh, ow, c), dtype=torch.uint8, device=level.device)
+        check(lib().astc_downsample(self._ctx, ctypes.byref(desc), ctypes.c_void_p(level.contiguous().data_ptr()),
+                                    ctypes.c_void_p(out.data_ptr()), _stream_handle(stream)))
+        return out
+
+    def mip_chain(self, base, stream=None):
+        """[base, level1, ...] while a side can halve, generated on the device."""
+        levels = [base]
+        while levels[-1].shape[0] >= 8 or levels[-1].shape[1] >= 8:
+            levels.append(self.downsample(levels[-1], stream))
+        return levels
+
+
+def mip_chain_info(desc):
+    n = ctypes.c_int()
+    check(lib().astc_mip_chain_info(ctypes.byref(desc), 0, None, ctypes.byref(n)))
+    arr = (ImageDesc * n.value)()
+    check(lib().astc_mip_chain_info(ctypes.byref(desc), n.value, arr, ctypes.byref(n)))
+    return list(arr)
+
+
+class Batch:
+    """astc_batch: many images encoded by one persistent kernel replayed as a graph.
+
+    images: list of (device tensor (H, W, C) uint8, block_dim, flags).  Output
+    tensors (one per image, (nblocks, 16) uint8) are allocated here and kept."""
+
+    def __init__(self, encoder, images, outputs=None):
+        import torch
+        n = len(images)
+        self._keep = []
+        descs = (ImageDesc * n)()
+        srcs = (ctypes.c_void_p * n)()
+        dsts = (ctypes.c_void_p * n)()
+        self.outputs = []
+        for i, (tex, dim, flags) in enumerate(images):
+            h, w, c = tex.shape
+            if tex.stride(2) != 1 or tex.stride(1) != c:
+                tex = tex.contiguous()
+            self._keep.append(tex)
+            descs[i] = make_desc(w, h, _CHANNELS_TO_FMT[c], dim, flags, row_pitch=tex.stride(0))
+            nb = output_size(descs[i]) // 16
+            out = outputs[i] if outputs is not None else torch.empty((nb, 16), dtype=torch.uint8, device=tex.device)
+            self.outputs.append(out)
+            srcs[i] = tex.data_ptr()
+            dsts[i] = out.data_ptr()
+        self._h = ctypes.c_void_p()
+        check(lib().astc_batch_create(encoder.handle, descs, srcs, dsts, n, ctypes.byref(self._h)))
+        self.encoder = encoder
+        self._descs, self._srcs, self._dsts, self._n = descs, srcs, dsts, n
+
+    def launch(self, stream=None):
+        """One replay of the batch's graph (the persistent kernel over every image)."""
+        check(lib().astc_batch_launch(self._h, _stream_handle(stream)))
+
+    def launch_plain(self, stream=None):
+        """The same images encoded by one plain launch each (astc_encode_each)."""
+        check(lib().astc_encode_each(self.encoder.handle, self._descs, self._srcs, self._dsts, self._n,
+                                     _stream_handle(stream)))
+
+    def autotune(self, reps=3, stream=None):
+        t = Tuning()
+        check(lib().astc_batch_autotune(self._h, _stream_handle(stream), int(reps), ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, blocks_per_lane, workgroups_per_cu, prefetch):
+        t = Tuning(blocks_per_lane, workgroups_per_cu, prefetch, 0)
+        check(lib().astc_batch_set_tuning(self._h, ctypes.byref(t)))
+
+    def tuning(self):
+        t = Tuning()
+        check(lib().astc_batch_get_tuning(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def info(self):
+        b, t, i = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32()
+        check(lib().astc_batch_info(self._h, ctypes.byref(b), ctypes.byref(t), ctypes.byref(i)))
+        return {"blocks": b.value, "texels": t.value, "items": i.value}
+
+    def close(self):
+        if self._h:
+            lib().astc_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 _default = {}

@@ -3,7 +3,7 @@
   sinusoid_rgb8(seed, w, h)  -> (h, w, 3) uint8   RGB, linear, 4x4
   noise_rgba8(seed, w, h)    -> (h, w, 4) uint8   RGBA, sRGB, 6x6
   normal_rg8(seed, w, h)     -> (h, w, 2) uint8   normal map, 4x4
-  mip_chain(img)             -> [level0, level1, ...] down to 4x4 (2x2 box filter)
+  mip_chain(img)             -> [level0, level1, ...] (2x2 box filter; power-of-two sides end at 4x4)
   texture_set(seed, n)       -> a list of mixed textures with their modes
 """
 import ctypes
@@ -33,7 +33,8 @@ def normal_rg8(seed, w, h):
 
 def downsample(img):
     h, w, c = img.shape
-    ow, oh = max(4, w >> 1), max(4, h >> 1)
+    ow = w >> 1 if w >= 8 else w
+    oh = h >> 1 if h >= 8 else h
     out = np.empty((oh, ow, c), np.uint8)
     nw, nh = ctypes.c_int(), ctypes.c_int()
     src = np.ascontiguousarray(img)
@@ -43,8 +44,9 @@ def downsample(img):
 
 
 def mip_chain(img):
+    """Levels while a side can halve (a side halves while it is >= 8)."""
     levels = [img]
-    while levels[-1].shape[0] > 4 or levels[-1].shape[1] > 4:
+    while levels[-1].shape[0] >= 8 or levels[-1].shape[1] >= 8:
         levels.append(downsample(levels[-1]))
     return levels
 

@@ -105,6 +105,59 @@ def time_plain(enc, wl, steps, warmup):
                 nblocks=nblocks, mtexels_s=texels * steps / wall / 1e6)
 
 
+def batched_vs_plain(enc, steps, warmup, n_textures=512):
+    """512 seeded textures (side 2^k, k in 6..12, square or 2:1; 60% sinusoid RGB8
+    4x4 / 25% noise RGBA8 sRGB 6x6 / 15% normal RG8 4x4), each with a full mip
+    chain generated on the device: one autotuned persistent-kernel graph replay
+    per step vs one plain launch per level (issued from C++)."""
+    import torch
+    import astc_encoder_amd as astc
+    from astc_encoder_amd import synth
+    t0 = time.perf_counter()
+    images = []
+    for mode, w, h, seed in synth.draw_texture_specs(0, n_textures):
+        gen, dim, flags, _ = synth.MODES[mode]
+        base = torch.from_numpy(gen(seed, w, h)).cuda()
+        for lvl in enc.mip_chain(base):
+            images.append((lvl, dim, flags))
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t0
+    b = astc.Batch(enc, images)
+    tuning = b.autotune(reps=2)
+    info = b.info()
+    stream = torch.cuda.current_stream()
+
+    def timed(fn, n):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t = time.perf_counter()
+        e0.record(stream)
+        for _ in range(n):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / n, e0.elapsed_time(e1) / n / 1e3
+
+    batched_wall, batched_gpu = timed(b.launch, steps)
+    snapshot = [o.clone() for o in b.outputs]
+    for o in b.outputs:
+        o.zero_()
+    plain_wall, plain_gpu = timed(b.launch_plain, max(3, steps // 4))
+    torch.cuda.synchronize()
+    same = all(torch.equal(a, o) for a, o in zip(snapshot, b.outputs))
+    b.close()
+    texels = info["texels"]
+    return {"workload": f"{n_textures} mixed textures with mip chains (seed 0)", "images": len(images),
+            "texels": texels, "blocks": info["blocks"], "work_items": info["items"], "tuning": tuning,
+            "batched_mtexels_s": round(texels / batched_wall / 1e6, 1),
+            "plain_mtexels_s": round(texels / plain_wall / 1e6, 1),
+            "batched_ms": round(batched_wall * 1e3, 3), "plain_ms": round(plain_wall * 1e3, 3),
+            "batched_vs_plain": round(plain_wall / batched_wall, 2), "bit_exact_batched_vs_plain": bool(same),
+            "generation_s": round(gen_s, 1)}
+
+
 def cpu_baseline(wl, img, gpu_blocks, budget_s=20.0):
     """The oracle (OpenMP over blocks, all cores of this host) on the same texture:
     one warm-up, then the median of up to 5 timed runs within the budget."""
@@ -134,6 +187,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="only the headline workload")
+    ap.add_argument("--no-batched", action="store_true", help="skip the mixed-texture batched vs plain run")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run 3 device copies of 256 MiB (a known-size kernel for scaling HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (textures are independent and "
@@ -144,6 +200,13 @@ def main():
     import astc_encoder_amd as astc
 
     enc = astc.Encoder(0)
+    if args.copy_calibration:
+        a = torch.empty(256 << 20, dtype=torch.uint8, device="cuda").fill_(1)
+        c = torch.empty_like(a)
+        for _ in range(3):
+            c.copy_(a)
+        torch.cuda.synchronize()
+        del a, c
     wls = workloads()
     head = wls["noise_rgba8_srgb_6x6"]
     r = time_plain(enc, head, args.steps, args.warmup)
@@ -187,6 +250,8 @@ def main():
             extras[key] = {"mtexels_s": round(e["mtexels_s"], 1), "kernel_ms": round(e["kernel_ms"], 4),
                            "valu_tflops": round(FLOPS_PER_BLOCK[wl["flops_key"]] * e["nblocks"] /
                                                 (e["kernel_ms"] * 1e-3) / 1e12, 2)}
+        if not args.no_batched:
+            extras["mixed_textures_batched_vs_plain"] = batched_vs_plain(enc, max(5, args.steps // 5), 2)
         line["extras"] = extras
     enc.close()
     print(json.dumps(line), flush=True)

@@ -81,6 +81,11 @@ int64_t astc_output_size(const astc_image_desc* desc);
  * One kernel launch on `stream`. */
 int astc_encode(astc_ctx* ctx, const astc_image_desc* desc, const void* src_dev, void* dst_dev, void* stream);
 
+/* The plain per-texture form over many images: astc_encode for each image in
+ * turn (one kernel launch per image, issued from C++), on `stream`. */
+int astc_encode_each(astc_ctx* ctx, const astc_image_desc* descs, const void* const* srcs, void* const* dsts,
+                     int count, void* stream);
+
 /* Write an .astc file: 16-byte header (magic 0x5CA1AB13, block dims, 24-bit
  * little-endian sizes, z = 1) followed by the blocks.  Replaces save_astc
  * (astc_save.h:52-76); unlike it, a failed open or short write is reported. */
@@ -90,6 +95,52 @@ int astc_write_file(const char* path, int block_dim_x, int block_dim_y, int widt
 /* Read an .astc file header; returns ASTC_OK and fills the fields. */
 int astc_read_file_header(const char* path, int* block_dim_x, int* block_dim_y, int* width, int* height,
                           int64_t* payload_bytes);
+
+/* ---- mip chains ------------------------------------------------------------
+ * Level l + 1 of a tightly packed image (row_pitch = width * bytes per texel):
+ * a side of at least 8 halves (rounding down), a shorter side is kept; 2x2 box
+ * filter in stored space, (a+b+c+d+2)>>2, or (a+b+1)>>1 along the one axis that
+ * halves.  The chain ends when neither side can halve (4x4 for power-of-two
+ * sides, i.e. max(4, w >> l) x max(4, h >> l)).
+ * No reference counterpart (the reference encodes one level, main.cpp:36). */
+
+/* Number of levels (including level 0) and, if `levels` is not NULL, each level's
+ * description (block size and flags copied from `base`, row_pitch tight). */
+int astc_mip_chain_info(const astc_image_desc* base, int max_levels, astc_image_desc* levels, int* count);
+/* Write level l+1 from level l (both tightly packed, device memory) on `stream`. */
+int astc_downsample(astc_ctx* ctx, const astc_image_desc* src_desc, const void* src_dev, void* dst_dev,
+                    void* stream);
+
+/* ---- batched persistent encoding ----------------------------------------
+ * Many images (textures, mip levels; any mix of block sizes, formats and
+ * flags) encoded by one persistent kernel fed by a device work list of block
+ * ranges, replayed as a captured hipGraph.  No reference counterpart: the
+ * reference encodes one texture per Dispatch (astc_encode.h:190); each image
+ * here gets exactly the blocks astc_encode would give it. */
+typedef struct astc_batch astc_batch;
+
+typedef struct astc_tuning {
+    int32_t blocks_per_lane;   /* 1, 2, 4 or 8: a work item is 256 x this many blocks of one image */
+    int32_t workgroups_per_cu; /* persistent 256-thread workgroups per compute unit, 1..4 */
+    int32_t prefetch;          /* 1: load the next block's texels while encoding the current one */
+    int32_t reserved;          /* 0 */
+} astc_tuning;
+
+/* descs/srcs/dsts: `count` images (device pointers as for astc_encode).  The
+ * batch keeps its own copy of the descriptions; buffers must outlive it. */
+int astc_batch_create(astc_ctx* ctx, const astc_image_desc* descs, const void* const* srcs, void* const* dsts,
+                      int count, astc_batch** out);
+/* Rebuild the work list and graph for a tuning (astc_batch_create uses defaults). */
+int astc_batch_set_tuning(astc_batch* batch, const astc_tuning* tuning);
+int astc_batch_get_tuning(const astc_batch* batch, astc_tuning* tuning);
+/* Time every candidate tuning (reps graph replays each, on `stream`, synchronous)
+ * and keep the fastest; writes it to *best if not NULL. */
+int astc_batch_autotune(astc_batch* batch, void* stream, int reps, astc_tuning* best);
+/* Encode every image of the batch: one graph replay on `stream`. */
+int astc_batch_launch(astc_batch* batch, void* stream);
+/* Totals: blocks, real texels (sum of width * height) and work items. */
+int astc_batch_info(const astc_batch* batch, int64_t* blocks, int64_t* texels, int32_t* items);
+int astc_batch_destroy(astc_batch* batch);
 
 /* Copy the library's sRGB table (256 floats: float(linear(c/255)) * 255.0f). */
 int astc_srgb_table(float out[256]);

@@ -1,0 +1,140 @@
+"""Batched persistent encoding and device mip chains.
+
+The batched kernel must give every image exactly the blocks of a plain launch
+(bit-exact, any tuning, every graph replay), and the device box filter must
+reproduce the host generator's mip levels byte for byte.
+"""
+import numpy as np
+import pytest
+
+import astc_encoder_amd as astc
+from astc_encoder_amd import synth
+from tests import oracle_ref as orc
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def enc():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    e = astc.Encoder(0)
+    yield e
+    e.close()
+
+
+@pytest.mark.parametrize("channels,w,h", [(4, 64, 64), (3, 256, 128), (2, 96, 40), (4, 7, 33), (3, 5, 4)])
+def test_device_mip_chain_matches_host(enc, channels, w, h):
+    rng = np.random.default_rng(w * h + channels)
+    img = rng.integers(0, 256, size=(h, w, channels), dtype=np.uint8)
+    host = synth.mip_chain(img)
+    dev = enc.mip_chain(torch.from_numpy(img).cuda())
+    torch.cuda.synchronize()
+    assert len(dev) == len(host)
+    for a, b in zip(dev, host):
+        np.testing.assert_array_equal(a.cpu().numpy(), b)
+    info = astc.mip_chain_info(astc.make_desc(w, h, {4: 0, 3: 1, 2: 2}[channels], 4, 0))
+    assert [(d.width, d.height) for d in info] == [(l.shape[1], l.shape[0]) for l in host]
+
+
+def mixed_images(enc, seed, n):
+    """n textures of the mixed distribution scaled down, each with its device mip chain."""
+    images = []
+    for mode, w, h, s in synth.draw_texture_specs(seed, n):
+        gen, dim, flags, fmt = synth.MODES[mode]
+        w, h = max(4, w // 16), max(4, h // 16)
+        base = torch.from_numpy(gen(s, w, h)).cuda()
+        for lvl in enc.mip_chain(base):
+            images.append((lvl, dim, flags))
+    return images
+
+
+def plain_outputs(enc, images):
+    outs = [enc.encode(t, block_dim=d, flags=f) for t, d, f in images]
+    torch.cuda.synchronize()
+    return [o.cpu().numpy() for o in outs]
+
+
+def test_batch_matches_plain_and_oracle(enc):
+    images = mixed_images(enc, 1, 40)
+    # odd shapes and an unaligned pitch among them
+    raw = torch.from_numpy(np.random.default_rng(0).integers(0, 256, (37, 23, 3), dtype=np.uint8)).cuda()
+    images += [(raw, 6, astc.FLAG_ALPHA), (raw[:, :21], 4, 0), (raw[:1, :1], 4, astc.FLAG_NORMAL)]
+    want = plain_outputs(enc, images)
+    b = astc.Batch(enc, images)
+    info = b.info()
+    assert info["blocks"] == sum(len(w) for w in want)
+    b.launch()
+    torch.cuda.synchronize()
+    for i, (o, w) in enumerate(zip(b.outputs, want)):
+        np.testing.assert_array_equal(o.cpu().numpy(), w, err_msg=f"image {i}")
+    # and against the oracle on a few of them
+    for i in (0, 1, len(images) - 3, len(images) - 1):
+        t, d, f = images[i]
+        np.testing.assert_array_equal(b.outputs[i].cpu().numpy(), orc.oracle_encode(t.cpu().numpy(), d, f))
+    b.close()
+
+
+@pytest.mark.parametrize("bpl,wpc,pf", [(1, 1, 0), (2, 2, 1), (4, 2, 0), (8, 1, 1), (8, 4, 1)])
+def test_every_tuning_gives_the_same_blocks(enc, bpl, wpc, pf):
+    images = mixed_images(enc, 2, 24)
+    want = plain_outputs(enc, images)
+    b = astc.Batch(enc, images)
+    b.set_tuning(bpl, wpc, pf)
+    assert b.tuning() == {"blocks_per_lane": bpl, "workgroups_per_cu": wpc, "prefetch": pf}
+    b.launch()
+    torch.cuda.synchronize()
+    for o, w in zip(b.outputs, want):
+        np.testing.assert_array_equal(o.cpu().numpy(), w)
+    b.close()
+
+
+def test_graph_replays_with_eager_work_between(enc):
+    images = mixed_images(enc, 3, 16)
+    want = plain_outputs(enc, images)
+    b = astc.Batch(enc, images)
+    for rep in range(4):
+        for o in b.outputs:
+            o.zero_()
+        torch.cuda.synchronize()
+        b.launch()
+        torch.cuda.synchronize()
+        for o, w in zip(b.outputs, want):
+            np.testing.assert_array_equal(o.cpu().numpy(), w, err_msg=f"replay {rep}")
+    b.close()
+
+
+def test_plain_form_over_a_batch(enc):
+    images = mixed_images(enc, 4, 12)
+    want = plain_outputs(enc, images)
+    b = astc.Batch(enc, images)
+    for o in b.outputs:
+        o.zero_()
+    b.launch_plain()
+    torch.cuda.synchronize()
+    for o, w in zip(b.outputs, want):
+        np.testing.assert_array_equal(o.cpu().numpy(), w)
+    b.close()
+
+
+def test_autotune_picks_a_valid_tuning(enc):
+    images = mixed_images(enc, 5, 16)
+    want = plain_outputs(enc, images)
+    b = astc.Batch(enc, images)
+    t = b.autotune(reps=2)
+    assert t["blocks_per_lane"] in (1, 2, 4, 8) and t["workgroups_per_cu"] in (1, 2)
+    for o in b.outputs:
+        o.zero_()
+    b.launch()
+    torch.cuda.synchronize()
+    for o, w in zip(b.outputs, want):
+        np.testing.assert_array_equal(o.cpu().numpy(), w)
+    b.close()
+
+
+def test_batch_rejects_bad_images(enc):
+    t = torch.zeros((8, 8, 4), dtype=torch.uint8, device="cuda")
+    with pytest.raises(astc.AstcError):
+        astc.Batch(enc, [(t, 5, 0)])
