@@ -44,6 +44,29 @@ class Tuning(ctypes.Structure):
                 "prefetch": self.prefetch}
 
 
+class ServerConfig(ctypes.Structure):
+    """astc_server_config"""
+    _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("max_inflight", ctypes.c_int32),
+                ("max_batch_requests", ctypes.c_int32), ("tuning", Tuning)]
+
+
+class Request(ctypes.Structure):
+    """astc_request"""
+    _fields_ = [("images", ctypes.POINTER(ImageDesc)), ("nimages", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("src_dev", ctypes.c_void_p), ("dst_dev", ctypes.c_void_p), ("arrival_s", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    """astc_server_stats"""
+    _fields_ = [("requests", ctypes.c_int64), ("images", ctypes.c_int64), ("texels", ctypes.c_int64),
+                ("blocks", ctypes.c_int64), ("batches", ctypes.c_int64), ("wall_s", ctypes.c_double),
+                ("mean_batch_requests", ctypes.c_double), ("peak_ring_bytes", ctypes.c_int64),
+                ("peak_inflight", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
 P = ctypes.c_void_p
 I32 = ctypes.c_int
 I64 = ctypes.c_int64
@@ -70,6 +93,11 @@ SIGNATURES = {
     "astc_batch_launch": (I32, [P, P]),
     "astc_batch_info": (I32, [P, ctypes.POINTER(I64), ctypes.POINTER(I64), ctypes.POINTER(I32)]),
     "astc_batch_destroy": (I32, [P]),
+    "astc_request_bytes": (I64, [PDESC, I32]),
+    "astc_server_create": (I32, [P, ctypes.POINTER(ServerConfig), ctypes.POINTER(P)]),
+    "astc_server_run_trace": (I32, [P, ctypes.POINTER(Request), I32, ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ServerStats)]),
+    "astc_server_destroy": (I32, [P]),
 }
 
 _lib = None

@@ -1,0 +1,425 @@
+// astc_server.cpp -- continuous batching of compression requests
+// (include/astc_mi355x.h, astc_server_*).
+//
+// One host thread runs the loop: poll the in-flight batches' events (complete
+// their requests, free their ring space), release requests whose arrival time
+// has passed, and whenever a batch slot is free form a batch from the waiting
+// requests that fit in the ring.  A batch is: one device-to-device copy of each
+// request's sources into the ring, one host-to-device copy of the batch's work
+// list into the slot's buffers, one replay of the slot's captured graph
+// (counter reset + persistent kernel), optional copies of the blocks to the
+// requests' destinations, and an event.  Slots use separate streams, so a batch
+// formed while others run overlaps with them on the device.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <new>
+#include <thread>
+#include <vector>
+
+#include "astc_internal.h"
+
+using namespace astc;
+
+namespace astc {
+void plan_work(const std::vector<BatchLevel>& levels, const std::vector<int>& modes, int blocks_per_lane,
+               std::vector<BatchSeg>* segs, std::vector<BatchItem>* items);
+}
+
+namespace {
+
+constexpr int64_t kAlign = 256;
+int64_t align_up(int64_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+// FIFO ring allocator over [0, size): allocations are made at the head and
+// released in any order; space is reclaimed from the oldest live allocation.
+class Ring {
+  public:
+    explicit Ring(int64_t size) : size_(size) {}
+    bool alloc(int64_t bytes, int64_t* off) {
+        if (bytes > size_) return false;
+        int64_t at;
+        if (live_.empty()) {
+            at = 0;
+        } else {
+            const int64_t tail = live_.front().off;
+            const bool wrapped = live_.back().off < tail;  // newest sits before the oldest
+            if (wrapped) {
+                if (head_ + bytes > tail) return false;
+                at = head_;
+            } else if (head_ + bytes <= size_) {
+                at = head_;
+            } else if (bytes <= tail) {
+                at = 0;
+            } else {
+                return false;
+            }
+        }
+        head_ = at + bytes;
+        live_.push_back({at, bytes, false});
+        used_ += bytes;
+        peak_ = std::max(peak_, used_);
+        *off = at;
+        return true;
+    }
+    void release(int64_t off) {
+        for (auto& a : live_)
+            if (a.off == off && !a.freed) {
+                a.freed = true;
+                used_ -= a.bytes;
+                break;
+            }
+        while (!live_.empty() && live_.front().freed) live_.pop_front();
+        if (live_.empty()) head_ = 0;
+    }
+    int64_t peak() const { return peak_; }
+
+  private:
+    struct A { int64_t off, bytes; bool freed; };
+    int64_t size_, head_ = 0, used_ = 0, peak_ = 0;
+    std::deque<A> live_;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    BatchLevel* d_levels = nullptr;
+    BatchSeg* d_segs = nullptr;
+    BatchItem* d_items = nullptr;
+    int* d_scalars = nullptr;  // [0] items, [1] queue head
+    BatchLevel* h_levels = nullptr;  // pinned staging
+    BatchSeg* h_segs = nullptr;
+    BatchItem* h_items = nullptr;
+    int* h_scalars = nullptr;
+    size_t cap_levels = 0, cap_segs = 0, cap_items = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool busy = false;
+    std::vector<int> reqs;
+    std::vector<int64_t> offs;
+};
+
+struct ReqGeom {
+    std::vector<BatchLevel> levels;  // pointers relative to the request's ring base
+    std::vector<int> modes;
+    int64_t src_bytes = 0, out_bytes = 0, texels = 0, blocks = 0;
+};
+
+}  // namespace
+
+struct astc_server {
+    astc_ctx* ctx = nullptr;
+    astc_server_config cfg{};
+    int num_cus = 0;
+    uint8_t* ring_base = nullptr;
+    std::vector<Slot> slots;
+};
+
+namespace {
+
+void free_slot_buffers(Slot& s) {
+    if (s.exec) (void)hipGraphExecDestroy(s.exec);
+    if (s.graph) (void)hipGraphDestroy(s.graph);
+    s.exec = nullptr;
+    s.graph = nullptr;
+    for (void* p : {(void*)s.d_levels, (void*)s.d_segs, (void*)s.d_items}) if (p) (void)hipFree(p);
+    for (void* p : {(void*)s.h_levels, (void*)s.h_segs, (void*)s.h_items}) if (p) (void)hipHostFree(p);
+    s.d_levels = nullptr; s.d_segs = nullptr; s.d_items = nullptr;
+    s.h_levels = nullptr; s.h_segs = nullptr; s.h_items = nullptr;
+    s.cap_levels = s.cap_segs = s.cap_items = 0;
+}
+
+// Make the slot's buffers hold at least the given counts; (re)capture its graph.
+int ensure_capacity(astc_server* sv, Slot& s, size_t nlev, size_t nseg, size_t nitem) {
+    if (nlev <= s.cap_levels && nseg <= s.cap_segs && nitem <= s.cap_items && s.exec) return ASTC_OK;
+    free_slot_buffers(s);
+    s.cap_levels = std::max<size_t>(std::max<size_t>(nlev, 64), 2 * nlev);
+    s.cap_segs = std::max<size_t>(std::max<size_t>(nseg, 256), 2 * nseg);
+    s.cap_items = std::max<size_t>(std::max<size_t>(nitem, 256), 2 * nitem);
+    hipError_t e;
+    if ((e = hipMalloc(&s.d_levels, s.cap_levels * sizeof(BatchLevel))) != hipSuccess ||
+        (e = hipMalloc(&s.d_segs, s.cap_segs * sizeof(BatchSeg))) != hipSuccess ||
+        (e = hipMalloc(&s.d_items, s.cap_items * sizeof(BatchItem))) != hipSuccess)
+        return hip_fail(e, "hipMalloc(work list)");
+    if ((e = hipHostMalloc(&s.h_levels, s.cap_levels * sizeof(BatchLevel), 0)) != hipSuccess ||
+        (e = hipHostMalloc(&s.h_segs, s.cap_segs * sizeof(BatchSeg), 0)) != hipSuccess ||
+        (e = hipHostMalloc(&s.h_items, s.cap_items * sizeof(BatchItem), 0)) != hipSuccess)
+        return hip_fail(e, "hipHostMalloc(staging)");
+    BatchArgs a;
+    a.levels = s.d_levels;
+    a.segs = s.d_segs;
+    a.items = s.d_items;
+    a.nitems = s.d_scalars;
+    a.counter = s.d_scalars + 1;
+    a.tables = sv->ctx->d_tables;
+    a.prefetch = sv->cfg.tuning.prefetch;
+    const int grid = std::max(1, sv->num_cus * sv->cfg.tuning.workgroups_per_cu);
+    if ((e = hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+        return hip_fail(e, "hipStreamBeginCapture");
+    hipError_t le = launch_encode_batch(a, grid, s.stream);
+    e = hipStreamEndCapture(s.stream, &s.graph);
+    if (le != hipSuccess) return hip_fail(le, "capture of the batch kernel");
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    if ((e = hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0)) != hipSuccess)
+        return hip_fail(e, "hipGraphInstantiate");
+    return ASTC_OK;
+}
+
+int geometry(const astc_request& r, ReqGeom* g) {
+    if (!r.images || r.nimages < 1 || !r.src_dev) return fail(ASTC_ERR_INVALID_ARG, "request without images");
+    g->levels.resize(r.nimages);
+    g->modes.resize(r.nimages);
+    int64_t so = 0, oo = 0;
+    for (int i = 0; i < r.nimages; ++i) {
+        ImageGeom ig;
+        int rc = check_desc(&r.images[i], &ig);
+        if (rc != ASTC_OK) return rc;
+        if (r.images[i].row_pitch != r.images[i].width * ig.bpp)
+            return fail(ASTC_ERR_INVALID_ARG, "request images must be tightly packed");
+        BatchLevel& L = g->levels[i];
+        L.f.src = reinterpret_cast<const uint8_t*>(so);  // offsets until placed in the ring
+        L.f.width = r.images[i].width;
+        L.f.height = r.images[i].height;
+        L.f.row_pitch = r.images[i].row_pitch;
+        L.f.fmt = r.images[i].src_format;
+        L.f.flags = r.images[i].flags;
+        L.f.aligned = (r.images[i].row_pitch & 3) == 0;
+        L.dst = reinterpret_cast<uint32_t*>(oo);
+        L.xblocks = ig.xblocks;
+        L.nblocks = (int)ig.nblocks;
+        g->modes[i] = mode_of(r.images[i].block_dim, r.images[i].flags);
+        so += align_up((int64_t)r.images[i].row_pitch * r.images[i].height);
+        oo += align_up(ig.nblocks * 16);
+        g->texels += (int64_t)r.images[i].width * r.images[i].height;
+        g->blocks += ig.nblocks;
+    }
+    g->src_bytes = so;
+    g->out_bytes = oo;
+    return ASTC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t astc_request_bytes(const astc_image_desc* images, int nimages) {
+    astc_request r{};
+    r.images = images;
+    r.nimages = nimages;
+    r.src_dev = reinterpret_cast<const void*>(1);
+    ReqGeom g;
+    int rc = geometry(r, &g);
+    if (rc != ASTC_OK) return rc;
+    return g.src_bytes + g.out_bytes;
+}
+
+int astc_server_create(astc_ctx* ctx, const astc_server_config* cfg, astc_server** out) {
+    if (!out) return fail(ASTC_ERR_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    if (!ctx || !cfg) return fail(ASTC_ERR_INVALID_ARG, "NULL argument");
+    const astc_tuning& t = cfg->tuning;
+    if (cfg->hbm_budget_bytes < kAlign || cfg->max_inflight < 1 || cfg->max_inflight > 8 ||
+        cfg->max_batch_requests < 1 ||
+        !(t.blocks_per_lane == 1 || t.blocks_per_lane == 2 || t.blocks_per_lane == 4 || t.blocks_per_lane == 8) ||
+        t.workgroups_per_cu < 1 || t.workgroups_per_cu > 4 || (t.prefetch != 0 && t.prefetch != 1))
+        return fail(ASTC_ERR_INVALID_ARG, "server configuration out of range");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    astc_server* sv = new (std::nothrow) astc_server();
+    if (!sv) return fail(ASTC_ERR_NO_MEMORY, "host allocation failed");
+    sv->ctx = ctx;
+    sv->cfg = *cfg;
+    int rc = ASTC_OK;
+    if ((e = hipDeviceGetAttribute(&sv->num_cus, hipDeviceAttributeMultiprocessorCount, ctx->device)) != hipSuccess)
+        rc = hip_fail(e, "hipDeviceGetAttribute");
+    else if ((e = hipMalloc(&sv->ring_base, cfg->hbm_budget_bytes)) != hipSuccess)
+        rc = fail(ASTC_ERR_NO_MEMORY, std::string("ring allocation failed: ") + hipGetErrorString(e));
+    if (rc == ASTC_OK) {
+        sv->slots.resize(cfg->max_inflight);
+        for (Slot& s : sv->slots) {
+            if ((e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking)) != hipSuccess ||
+                (e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming)) != hipSuccess ||
+                (e = hipMalloc(&s.d_scalars, 2 * sizeof(int))) != hipSuccess ||
+                (e = hipHostMalloc(&s.h_scalars, 2 * sizeof(int), 0)) != hipSuccess) {
+                rc = hip_fail(e, "slot setup");
+                break;
+            }
+            if ((rc = ensure_capacity(sv, s, 64, 256, 256)) != ASTC_OK) break;
+        }
+    }
+    if (rc != ASTC_OK) {
+        astc_server_destroy(sv);
+        return rc;
+    }
+    *out = sv;
+    return ASTC_OK;
+}
+
+int astc_server_destroy(astc_server* sv) {
+    if (!sv) return ASTC_OK;
+    (void)hipSetDevice(sv->ctx->device);
+    for (Slot& s : sv->slots) {
+        if (s.stream) (void)hipStreamSynchronize(s.stream);
+        free_slot_buffers(s);
+        if (s.d_scalars) (void)hipFree(s.d_scalars);
+        if (s.h_scalars) (void)hipHostFree(s.h_scalars);
+        if (s.done) (void)hipEventDestroy(s.done);
+        if (s.stream) (void)hipStreamDestroy(s.stream);
+    }
+    if (sv->ring_base) (void)hipFree(sv->ring_base);
+    delete sv;
+    return ASTC_OK;
+}
+
+int astc_server_run_trace(astc_server* sv, const astc_request* reqs, int n, double* latency_s,
+                          astc_server_stats* stats) {
+    if (!sv || (!reqs && n > 0) || !latency_s || n < 0) return fail(ASTC_ERR_INVALID_ARG, "NULL argument");
+    hipError_t e = hipSetDevice(sv->ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    std::vector<ReqGeom> geo(n);
+    astc_server_stats st{};
+    for (int i = 0; i < n; ++i) {
+        int rc = geometry(reqs[i], &geo[i]);
+        if (rc != ASTC_OK) return fail(rc, "request " + std::to_string(i) + ": " + astc_last_error());
+        if (i && reqs[i].arrival_s < reqs[i - 1].arrival_s)
+            return fail(ASTC_ERR_INVALID_ARG, "arrival times must be non-decreasing");
+        if (geo[i].src_bytes + geo[i].out_bytes > sv->cfg.hbm_budget_bytes)
+            return fail(ASTC_ERR_BUDGET, "request " + std::to_string(i) + " is larger than the HBM budget");
+        st.images += reqs[i].nimages;
+        st.texels += geo[i].texels;
+        st.blocks += geo[i].blocks;
+    }
+    st.requests = n;
+    Ring ring(sv->cfg.hbm_budget_bytes);
+    std::deque<int> waiting;
+    int next = 0, done = 0, inflight = 0;
+    int64_t batched_reqs = 0;
+    std::vector<BatchLevel> levels;
+    std::vector<int> modes;
+    std::vector<BatchSeg> segs;
+    std::vector<BatchItem> items;
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    auto now = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
+    double last_done = 0.0;
+    int rc = ASTC_OK;
+
+    while (done < n && rc == ASTC_OK) {
+        bool progressed = false;
+        // completions
+        for (Slot& s : sv->slots) {
+            if (!s.busy) continue;
+            e = hipEventQuery(s.done);
+            if (e == hipErrorNotReady) continue;
+            if (e != hipSuccess) {
+                rc = hip_fail(e, "batch event");
+                break;
+            }
+            const double t = now();
+            for (size_t k = 0; k < s.reqs.size(); ++k) {
+                latency_s[s.reqs[k]] = t - reqs[s.reqs[k]].arrival_s;
+                ring.release(s.offs[k]);
+            }
+            done += (int)s.reqs.size();
+            last_done = t;
+            s.busy = false;
+            --inflight;
+            progressed = true;
+        }
+        if (rc != ASTC_OK) break;
+        // arrivals
+        const double t = now();
+        while (next < n && reqs[next].arrival_s <= t) waiting.push_back(next++);
+        // launches
+        for (Slot& s : sv->slots) {
+            if (s.busy || waiting.empty()) continue;
+            s.reqs.clear();
+            s.offs.clear();
+            levels.clear();
+            modes.clear();
+            while (!waiting.empty() && (int)s.reqs.size() < sv->cfg.max_batch_requests) {
+                const int r = waiting.front();
+                int64_t off;
+                if (!ring.alloc(geo[r].src_bytes + geo[r].out_bytes, &off)) break;
+                waiting.pop_front();
+                s.reqs.push_back(r);
+                s.offs.push_back(off);
+                uint8_t* base = sv->ring_base + off;
+                for (size_t i = 0; i < geo[r].levels.size(); ++i) {
+                    BatchLevel L = geo[r].levels[i];
+                    L.f.src = base + reinterpret_cast<uintptr_t>(L.f.src);
+                    L.dst = reinterpret_cast<uint32_t*>(base + geo[r].src_bytes + reinterpret_cast<uintptr_t>(L.dst));
+                    levels.push_back(L);
+                    modes.push_back(geo[r].modes[i]);
+                }
+            }
+            if (s.reqs.empty()) break;  // the ring is full: wait for completions
+            plan_work(levels, modes, sv->cfg.tuning.blocks_per_lane, &segs, &items);
+            if (levels.size() > s.cap_levels || segs.size() > s.cap_segs || items.size() > s.cap_items) {
+                // grow every idle slot at once: regrowing (allocation + graph capture)
+                // costs milliseconds, so it should not recur slot by slot mid-stream
+                for (Slot& o : sv->slots)
+                    if (!o.busy && rc == ASTC_OK)
+                        rc = ensure_capacity(sv, o, std::max(levels.size(), o.cap_levels),
+                                             std::max(segs.size(), o.cap_segs), std::max(items.size(), o.cap_items));
+                if (rc != ASTC_OK) break;
+            }
+            std::memcpy(s.h_levels, levels.data(), levels.size() * sizeof(BatchLevel));
+            std::memcpy(s.h_segs, segs.data(), segs.size() * sizeof(BatchSeg));
+            std::memcpy(s.h_items, items.data(), items.size() * sizeof(BatchItem));
+            s.h_scalars[0] = (int)items.size();
+            s.h_scalars[1] = 0;
+            e = hipSuccess;
+            for (size_t k = 0; k < s.reqs.size() && e == hipSuccess; ++k) {
+                const int r = s.reqs[k];
+                e = hipMemcpyAsync(sv->ring_base + s.offs[k], reqs[r].src_dev, geo[r].src_bytes,
+                                   hipMemcpyDeviceToDevice, s.stream);
+            }
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(s.d_levels, s.h_levels, levels.size() * sizeof(BatchLevel), hipMemcpyHostToDevice,
+                                   s.stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(s.d_segs, s.h_segs, segs.size() * sizeof(BatchSeg), hipMemcpyHostToDevice,
+                                   s.stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(s.d_items, s.h_items, items.size() * sizeof(BatchItem), hipMemcpyHostToDevice,
+                                   s.stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(s.d_scalars, s.h_scalars, 2 * sizeof(int), hipMemcpyHostToDevice, s.stream);
+            if (e == hipSuccess) e = hipGraphLaunch(s.exec, s.stream);
+            for (size_t k = 0; k < s.reqs.size() && e == hipSuccess; ++k) {
+                const int r = s.reqs[k];
+                if (reqs[r].dst_dev)
+                    e = hipMemcpyAsync(reqs[r].dst_dev, sv->ring_base + s.offs[k] + geo[r].src_bytes,
+                                       geo[r].out_bytes, hipMemcpyDeviceToDevice, s.stream);
+            }
+            if (e == hipSuccess) e = hipEventRecord(s.done, s.stream);
+            if (e != hipSuccess) {
+                rc = hip_fail(e, "batch submission");
+                break;
+            }
+            s.busy = true;
+            ++inflight;
+            ++st.batches;
+            batched_reqs += (int64_t)s.reqs.size();
+            st.peak_inflight = std::max(st.peak_inflight, inflight);
+            progressed = true;
+        }
+        if (rc != ASTC_OK) break;
+        // idle until the next arrival when nothing is on the GPU
+        if (!progressed && inflight == 0 && waiting.empty() && next < n) {
+            const double gap = reqs[next].arrival_s - now();
+            if (gap > 2e-4) std::this_thread::sleep_for(std::chrono::duration<double>(gap - 1e-4));
+        }
+    }
+    for (Slot& s : sv->slots) (void)hipStreamSynchronize(s.stream);
+    if (rc != ASTC_OK) return rc;
+    st.wall_s = n ? last_done - reqs[0].arrival_s : 0.0;
+    st.mean_batch_requests = st.batches ? double(batched_reqs) / st.batches : 0.0;
+    st.peak_ring_bytes = ring.peak();
+    if (stats) *stats = st;
+    return ASTC_OK;
+}
+
+}  // extern "C"

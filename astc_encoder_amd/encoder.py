@@ -14,7 +14,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from ._capi import (FLAG_ALPHA, FLAG_NORMAL, FLAG_SRGB, FMT_RG8, FMT_RGB8, FMT_RGBA8, ImageDesc,
-                    BYTES_PER_TEXEL, Tuning, check, lib)
+                    BYTES_PER_TEXEL, Request, ServerConfig, ServerStats, Tuning, check, lib)
 
 _CHANNELS_TO_FMT = {4: FMT_RGBA8, 3: FMT_RGB8, 2: FMT_RG8}
 
@@ -206,6 +206,80 @@ class Batch:
     def close(self):
         if self._h:
             lib().astc_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _align256(n):
+    return (n + 255) // 256 * 256
+
+
+class PackedImages:
+    """A request's images (e.g. one mip chain) packed back to back in one device
+    buffer, image i at offset sum_{j<i} align256(bytes_j): the layout
+    astc_server_run_trace expects.  Also sizes the matching block layout."""
+
+    def __init__(self, tensors, block_dim, flags):
+        import torch
+        n = len(tensors)
+        self.descs = (ImageDesc * n)()
+        offs, total, out_offs, out_total = [], 0, [], 0
+        for i, t in enumerate(tensors):
+            h, w, c = t.shape
+            self.descs[i] = make_desc(w, h, _CHANNELS_TO_FMT[c], block_dim, flags)
+            offs.append(total)
+            total += _align256(h * w * c)
+            out_offs.append(out_total)
+            out_total += _align256(output_size(self.descs[i]))
+        self.buffer = torch.empty(max(total, 256), dtype=torch.uint8, device=tensors[0].device)
+        for t, o in zip(tensors, offs):
+            self.buffer[o:o + t.numel()].copy_(t.contiguous().view(-1))
+        self.nimages = n
+        self.src_offsets = offs
+        self.out_offsets = out_offs
+        self.out_bytes = out_total
+        self.texels = sum(int(t.shape[0]) * int(t.shape[1]) for t in tensors)
+        self.block_counts = [output_size(d) // 16 for d in self.descs]
+
+    def blocks_of(self, out, i):
+        """Image i's (nblocks, 16) blocks from a destination buffer of out_bytes."""
+        o = self.out_offsets[i]
+        return out[o:o + self.block_counts[i] * 16].view(-1, 16)
+
+
+class Server:
+    """astc_server: continuous batching of requests under an HBM budget."""
+
+    def __init__(self, encoder, hbm_budget_bytes, max_inflight=3, max_batch_requests=64,
+                 tuning=(4, 2, 1)):
+        cfg = ServerConfig(int(hbm_budget_bytes), int(max_inflight), int(max_batch_requests),
+                           Tuning(tuning[0], tuning[1], tuning[2], 0))
+        self._h = ctypes.c_void_p()
+        check(lib().astc_server_create(encoder.handle, ctypes.byref(cfg), ctypes.byref(self._h)))
+
+    def run_trace(self, requests):
+        """requests: list of (PackedImages, arrival_s, dst tensor or None), arrival
+        times non-decreasing.  Runs in real time on this thread; returns (latency
+        seconds per request as a numpy array, stats dict)."""
+        n = len(requests)
+        arr = (Request * n)()
+        for i, (p, t, dst) in enumerate(requests):
+            arr[i] = Request(p.descs, p.nimages, 0, ctypes.c_void_p(p.buffer.data_ptr()),
+                             ctypes.c_void_p(dst.data_ptr() if dst is not None else 0), float(t))
+        lat = np.zeros(n, np.float64)
+        st = ServerStats()
+        check(lib().astc_server_run_trace(self._h, arr, n, lat.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                          ctypes.byref(st)))
+        return lat, st.as_dict()
+
+    def close(self):
+        if self._h:
+            lib().astc_server_destroy(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):

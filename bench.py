@@ -158,6 +158,42 @@ def batched_vs_plain(enc, steps, warmup, n_textures=512):
             "generation_s": round(gen_s, 1)}
 
 
+def serving_sweep(enc, rates=(1000, 2000, 5000, 10000, 20000, 50000), seconds=0.5, pool_size=64):
+    """Poisson request streams at each rate; every request is one mip-chained
+    texture drawn from the mixed distribution (a pool of device-resident sources,
+    copied into the server's ring on admission).  64 GB HBM budget."""
+    import torch
+    import astc_encoder_amd as astc
+    from astc_encoder_amd import synth
+    pool = []
+    for mode, w, h, seed in synth.draw_texture_specs(1, pool_size):
+        gen, dim, flags, _ = synth.MODES[mode]
+        levels = enc.mip_chain(torch.from_numpy(gen(seed, w, h)).cuda())
+        pool.append(astc.PackedImages(levels, dim, flags))
+        del levels
+    torch.cuda.synchronize()
+    srv = astc.Server(enc, 64 << 30, max_inflight=3, max_batch_requests=256)
+    rng = np.random.default_rng(0)
+    warm = [(pool[i % pool_size], 0.0, None) for i in range(2 * pool_size)]
+    srv.run_trace(warm)
+    results = []
+    for lam in rates:
+        n = max(200, int(lam * seconds))
+        t = np.cumsum(rng.exponential(1.0 / lam, n))
+        picks = rng.integers(0, pool_size, n)
+        lat, st = srv.run_trace([(pool[k], float(a), None) for k, a in zip(picks, t)])
+        keep = lat[n // 10:]
+        results.append({"rate_per_s": lam, "requests": n, "p50_ms": round(float(np.percentile(keep, 50)) * 1e3, 3),
+                        "p99_ms": round(float(np.percentile(keep, 99)) * 1e3, 3),
+                        "mtexels_s": round(st["texels"] / st["wall_s"] / 1e6, 1),
+                        "mean_texels_per_request": int(st["texels"] / n),
+                        "batches": st["batches"], "mean_batch_requests": round(st["mean_batch_requests"], 2),
+                        "peak_inflight": st["peak_inflight"]})
+    srv.close()
+    return {"workload": f"Poisson arrivals, {pool_size} mip-chained textures of the mixed distribution (seed 1)",
+            "hbm_budget_gb": 64, "max_inflight_batches": 3, "sweep": results}
+
+
 def cpu_baseline(wl, img, gpu_blocks, budget_s=20.0):
     """The oracle (OpenMP over blocks, all cores of this host) on the same texture:
     one warm-up, then the median of up to 5 timed runs within the budget."""
@@ -188,6 +224,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="only the headline workload")
     ap.add_argument("--no-batched", action="store_true", help="skip the mixed-texture batched vs plain run")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving sweep")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run 3 device copies of 256 MiB (a known-size kernel for scaling HBM counters)")
     args = ap.parse_args()
@@ -252,6 +289,8 @@ def main():
                                                 (e["kernel_ms"] * 1e-3) / 1e12, 2)}
         if not args.no_batched:
             extras["mixed_textures_batched_vs_plain"] = batched_vs_plain(enc, max(5, args.steps // 5), 2)
+        if not args.no_serving:
+            extras["serving"] = serving_sweep(enc)
         line["extras"] = extras
     enc.close()
     print(json.dumps(line), flush=True)

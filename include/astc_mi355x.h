@@ -142,6 +142,55 @@ int astc_batch_launch(astc_batch* batch, void* stream);
 int astc_batch_info(const astc_batch* batch, int64_t* blocks, int64_t* texels, int32_t* items);
 int astc_batch_destroy(astc_batch* batch);
 
+/* ---- serving: continuous batching of compression requests ------------------
+ * A request is a set of images (typically one texture's mip chain) whose
+ * tightly packed sources lie back to back in device memory, image i at offset
+ * sum over j < i of align256(width_j * height_j * bytes per texel).  The server
+ * owns a device ring of `hbm_budget_bytes`: an admitted request's sources are
+ * copied into it (one device-to-device copy) next to space for its blocks; a
+ * request waits while the ring lacks room.  Up to `max_inflight` batches run at
+ * once, each on its own stream, each one replay of the persistent kernel's
+ * graph over every image of every request in it; a new batch is formed from
+ * whatever has arrived as soon as a slot is free, so work that arrives while
+ * other batches are on the GPU overlaps with them.  No reference counterpart. */
+typedef struct astc_server astc_server;
+
+typedef struct astc_server_config {
+    int64_t hbm_budget_bytes;   /* ring size (sources + blocks of admitted requests) */
+    int32_t max_inflight;       /* batches on the GPU at once, 1..8 */
+    int32_t max_batch_requests; /* requests per batch, >= 1 */
+    astc_tuning tuning;         /* persistent-kernel tuning (blocks_per_lane, workgroups_per_cu, prefetch) */
+} astc_server_config;
+
+typedef struct astc_request {
+    const astc_image_desc* images; /* nimages descriptions (row_pitch = width * bytes per texel) */
+    int32_t nimages;
+    int32_t reserved;
+    const void* src_dev;           /* the images' texels, laid out as described above */
+    void* dst_dev;                 /* NULL, or where the blocks (image after image) are copied on completion */
+    double arrival_s;              /* arrival time, seconds after the start of the trace */
+} astc_request;
+
+typedef struct astc_server_stats {
+    int64_t requests, images, texels, blocks, batches;
+    double wall_s;               /* first arrival to last completion */
+    double mean_batch_requests;
+    int64_t peak_ring_bytes;     /* most ring bytes held at once */
+    int32_t peak_inflight;       /* most batches on the GPU at once */
+    int32_t reserved;
+} astc_server_stats;
+
+/* Bytes the ring holds for one request (sources + blocks, 256-byte aligned). */
+int64_t astc_request_bytes(const astc_image_desc* images, int nimages);
+int astc_server_create(astc_ctx* ctx, const astc_server_config* config, astc_server** out);
+/* Serve `n` requests (arrival_s non-decreasing) in real time on the calling
+ * thread: each request is released at its arrival time, admitted when the ring
+ * has room, batched, and completed; latency_s[i] = completion - arrival (host
+ * steady clock, completion observed by polling the batch's event). */
+int astc_server_run_trace(astc_server* server, const astc_request* requests, int n, double* latency_s,
+                          astc_server_stats* stats);
+int astc_server_destroy(astc_server* server);
+
 /* Copy the library's sRGB table (256 floats: float(linear(c/255)) * 255.0f). */
 int astc_srgb_table(float out[256]);
 

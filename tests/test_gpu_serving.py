@@ -1,0 +1,102 @@
+"""Serving: continuous batching of requests in a device ring under an HBM budget.
+
+Every request's blocks must equal a plain encode of its images whether it is
+served alone or among many concurrent requests (the kernel is deterministic per
+block), batches must actually overlap and combine requests, and admission must
+respect the ring budget.
+"""
+import numpy as np
+import pytest
+
+import astc_encoder_amd as astc
+from astc_encoder_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def enc():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    e = astc.Encoder(0)
+    yield e
+    e.close()
+
+
+@pytest.fixture(scope="module")
+def pool(enc):
+    """Six small mip-chained textures covering the three modes."""
+    out = []
+    for mode, w, h, s in synth.draw_texture_specs(7, 6):
+        gen, dim, flags, _ = synth.MODES[mode]
+        w, h = max(8, w // 8), max(8, h // 8)
+        levels = enc.mip_chain(torch.from_numpy(gen(s, w, h)).cuda())
+        want = [enc.encode(l, block_dim=dim, flags=flags) for l in levels]
+        out.append((astc.PackedImages(levels, dim, flags), want))
+    torch.cuda.synchronize()
+    return out
+
+
+def check_request(packed, want, dst):
+    for i, w in enumerate(want):
+        np.testing.assert_array_equal(packed.blocks_of(dst, i).cpu().numpy(), w.cpu().numpy())
+
+
+def test_alone_and_in_the_mix_give_the_same_blocks(enc, pool):
+    srv = astc.Server(enc, 256 << 20, max_inflight=3, max_batch_requests=16)
+    # alone
+    p, want = pool[2]
+    dst = torch.zeros(p.out_bytes, dtype=torch.uint8, device="cuda")
+    lat, st = srv.run_trace([(p, 0.0, dst)])
+    assert st["requests"] == 1 and st["batches"] == 1 and lat[0] > 0
+    check_request(p, want, dst)
+    alone = dst.clone()
+    # in a mix: 300 requests arriving at ~100k/s, so batches form and overlap
+    rng = np.random.default_rng(0)
+    t = np.cumsum(rng.exponential(1e-5, 300))
+    picks = rng.integers(0, len(pool), 300)
+    picks[[0, 150, 299]] = 2
+    reqs, dsts = [], []
+    for k in range(300):
+        pk, _ = pool[picks[k]]
+        d = torch.zeros(pk.out_bytes, dtype=torch.uint8, device="cuda")
+        reqs.append((pk, float(t[k]), d))
+        dsts.append(d)
+    lat, st = srv.run_trace(reqs)
+    torch.cuda.synchronize()
+    assert st["requests"] == 300 and np.all(lat > 0)
+    assert st["batches"] < 300 and st["mean_batch_requests"] > 1.0
+    assert st["peak_inflight"] >= 2
+    for k in range(300):
+        pk, want = pool[picks[k]]
+        check_request(pk, want, dsts[k])
+    for k in (0, 150, 299):
+        assert torch.equal(dsts[k], alone)
+    srv.close()
+
+
+def test_admission_waits_for_ring_space(enc, pool):
+    biggest = max(p.buffer.numel() + p.out_bytes for p, _ in pool)
+    budget = biggest + 4096
+    srv = astc.Server(enc, budget, max_inflight=2, max_batch_requests=8)
+    reqs = []
+    for k in range(40):
+        pk, _ = pool[k % len(pool)]
+        reqs.append((pk, 0.0, torch.zeros(pk.out_bytes, dtype=torch.uint8, device="cuda")))
+    lat, st = srv.run_trace(reqs)
+    torch.cuda.synchronize()
+    assert st["peak_ring_bytes"] <= budget
+    for k, (pk, _, d) in enumerate(reqs):
+        check_request(pk, pool[k % len(pool)][1], d)
+    srv.close()
+
+
+def test_oversized_request_is_rejected(enc, pool):
+    pk, _ = pool[0]
+    srv = astc.Server(enc, 1024, max_inflight=1, max_batch_requests=1)
+    with pytest.raises(astc.AstcError) as e:
+        srv.run_trace([(pk, 0.0, None)])
+    assert e.value.code == -6
+    srv.close()
