@@ -10,6 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(HERE, "_lib")
 LIB_PATH = os.path.join(LIB_DIR, "libastc_mi355x.so")
 SYNTH_PATH = os.path.join(LIB_DIR, "libastc_synth.so")
+CLI_PATH = os.path.join(LIB_DIR, "astc_cs_enc")
 
 ASTC_OK = 0
 ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "out of memory", -4: "unsupported",
@@ -79,11 +80,15 @@ SIGNATURES = {
     "astc_ctx_destroy": (I32, [P]),
     "astc_output_size": (I64, [ctypes.POINTER(ImageDesc)]),
     "astc_encode": (I32, [P, ctypes.POINTER(ImageDesc), P, P, P]),
+    "astc_decode": (I32, [P, I32, I32, I32, I32, P, P, P]),
     "astc_encode_each": (I32, [P, PDESC, ctypes.POINTER(P), ctypes.POINTER(P), I32, P]),
     "astc_write_file": (I32, [ctypes.c_char_p, I32, I32, I32, I32, P, I64]),
     "astc_read_file_header": (I32, [ctypes.c_char_p, ctypes.POINTER(I32), ctypes.POINTER(I32),
                                     ctypes.POINTER(I32), ctypes.POINTER(I32), ctypes.POINTER(I64)]),
     "astc_srgb_table": (I32, [P]),
+    "astc_load_image": (I32, [ctypes.c_char_p, I32, ctypes.POINTER(I32), ctypes.POINTER(I32),
+                              ctypes.POINTER(ctypes.POINTER(ctypes.c_uint8))]),
+    "astc_free_image": (None, [ctypes.POINTER(ctypes.c_uint8)]),
     "astc_mip_chain_info": (I32, [PDESC, I32, PDESC, ctypes.POINTER(I32)]),
     "astc_downsample": (I32, [P, PDESC, P, P, P]),
     "astc_batch_create": (I32, [P, PDESC, ctypes.POINTER(P), ctypes.POINTER(P), I32, ctypes.POINTER(P)]),

@@ -89,10 +89,12 @@ int astc_ctx_create(int device, astc_ctx** out) {
         delete ctx;
         return hip_fail(e, "hipMalloc(tables)");
     }
-    if ((e = hipMemcpy(ctx->d_tables, &ctx->h_tables, sizeof(DeviceTables), hipMemcpyHostToDevice)) != hipSuccess) {
-        (void)hipFree(ctx->d_tables);
-        delete ctx;
-        return hip_fail(e, "hipMemcpy(tables)");
+    if ((e = hipMemcpy(ctx->d_tables, &ctx->h_tables, sizeof(DeviceTables), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMalloc(&ctx->d_dec_tables, sizeof(TritQuintDecodeTable))) != hipSuccess ||
+        (e = hipMemcpy(ctx->d_dec_tables, &kDecodeTables, sizeof(TritQuintDecodeTable), hipMemcpyHostToDevice)) !=
+            hipSuccess) {
+        astc_ctx_destroy(ctx);
+        return hip_fail(e, "context tables");
     }
     *out = ctx;
     return ASTC_OK;
@@ -102,6 +104,7 @@ int astc_ctx_destroy(astc_ctx* ctx) {
     if (!ctx) return ASTC_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->d_tables) (void)hipFree(ctx->d_tables);
+    if (ctx->d_dec_tables) (void)hipFree(ctx->d_dec_tables);
     delete ctx;
     return ASTC_OK;
 }
@@ -124,6 +127,25 @@ int astc_encode(astc_ctx* ctx, const astc_image_desc* desc, const void* src_dev,
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     e = launch_encode_plain(ctx->d_tables, *desc, g, src_dev, dst_dev, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(e, "encode launch");
+    return ASTC_OK;
+}
+
+int astc_decode(astc_ctx* ctx, int width, int height, int block_dim_x, int block_dim_y, const void* blocks_dev,
+                void* rgba_dev, void* stream) {
+    if (!ctx || !blocks_dev || !rgba_dev) return fail(ASTC_ERR_INVALID_ARG, "NULL argument");
+    if (width < 1 || height < 1 || width > (1 << 24) - 1 || height > (1 << 24) - 1)
+        return fail(ASTC_ERR_INVALID_ARG, "width/height must be in 1..16777215");
+    if (block_dim_x < 4 || block_dim_x > 12 || block_dim_y < 4 || block_dim_y > 12)
+        return fail(ASTC_ERR_INVALID_ARG, "block dims must be 4..12");
+    if ((reinterpret_cast<uintptr_t>(blocks_dev) & 15u) || (reinterpret_cast<uintptr_t>(rgba_dev) & 3u))
+        return fail(ASTC_ERR_INVALID_ARG, "blocks must be 16-byte and rgba 4-byte aligned");
+    const int64_t nb = (int64_t)((width + block_dim_x - 1) / block_dim_x) * ((height + block_dim_y - 1) / block_dim_y);
+    if (nb > (int64_t)0x7FFFFF00) return fail(ASTC_ERR_UNSUPPORTED, "more than 2^31-256 blocks in one image");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    e = launch_decode(ctx->d_dec_tables, blocks_dev, rgba_dev, width, height, block_dim_x, block_dim_y,
+                      static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "decode launch");
     return ASTC_OK;
 }
 

@@ -84,6 +84,59 @@ constexpr TritEncodeTable make_trit_encode_table() {
 }
 constexpr TritEncodeTable kTritEncode = make_trit_encode_table();
 
+// ---- specification quint decoder (7 bits -> 3 quints) ----
+struct Quints3 { int q[3]; };
+constexpr Quints3 decode_quint_block(uint32_t Q) {
+    Quints3 r{};
+    const int q0b = bits_of(Q, 0, 0);
+    if (bits_of(Q, 2, 1) == 3 && bits_of(Q, 6, 5) == 0) {
+        r.q[2] = (q0b << 2) | ((bits_of(Q, 4, 4) & (1 - q0b)) << 1) | (bits_of(Q, 3, 3) & (1 - q0b));
+        r.q[1] = 4;
+        r.q[0] = 4;
+        return r;
+    }
+    int C = 0;
+    if (bits_of(Q, 2, 1) == 3) {
+        r.q[2] = 4;
+        C = (bits_of(Q, 4, 3) << 3) | ((3 - bits_of(Q, 6, 5)) << 1) | q0b;
+    } else {
+        r.q[2] = bits_of(Q, 6, 5);
+        C = bits_of(Q, 4, 0);
+    }
+    if ((C & 7) == 5) {
+        r.q[1] = 4;
+        r.q[0] = (C >> 3) & 3;
+    } else {
+        r.q[1] = (C >> 3) & 3;
+        r.q[0] = C & 7;
+    }
+    return r;
+}
+
+// Decoder tables: packed 8-bit trit code -> 5 trits (2 bits each), packed 7-bit
+// quint code -> 3 quints (3 bits each).
+struct TritQuintDecodeTable { uint16_t trits[256]; uint16_t quints[128]; };
+constexpr TritQuintDecodeTable make_decode_tables() {
+    TritQuintDecodeTable t{};
+    for (uint32_t T = 0; T < 256; ++T) {
+        Trits5 d = decode_trit_block(T);
+        t.trits[T] = uint16_t(d.t[0] | d.t[1] << 2 | d.t[2] << 4 | d.t[3] << 6 | d.t[4] << 8);
+    }
+    for (uint32_t Q = 0; Q < 128; ++Q) {
+        Quints3 d = decode_quint_block(Q);
+        t.quints[Q] = uint16_t(d.q[0] | d.q[1] << 3 | d.q[2] << 6);
+    }
+    return t;
+}
+constexpr TritQuintDecodeTable kDecodeTables = make_decode_tables();
+
+// (levels, trits, quints, bits) of the 21 quantisation methods (ASTC_Encode.hlsl:47-68)
+struct QuantLevel { uint8_t levels_minus1, trits, quints, bits; };
+constexpr QuantLevel kQuantLevels[21] = {
+    {1, 0, 0, 1},   {2, 1, 0, 0},   {3, 0, 0, 2},   {4, 0, 1, 0},   {5, 1, 0, 1},   {7, 0, 0, 3},   {9, 0, 1, 1},
+    {11, 1, 0, 2},  {15, 0, 0, 4},  {19, 0, 1, 2},  {23, 1, 0, 3},  {31, 0, 0, 5},  {39, 0, 1, 3},  {47, 1, 0, 4},
+    {63, 0, 0, 6},  {79, 0, 1, 4},  {95, 1, 0, 5},  {127, 0, 0, 7}, {159, 0, 1, 5}, {191, 1, 0, 6}, {255, 0, 0, 8}};
+
 // ---- weight unquantisation for the two trit-coded weight ranges in use ----
 // QUANT_6: 1 trit + 1 bit, C = 50; QUANT_12: 1 trit + 2 bits, C = 23, B = b000b0b
 // (ASTC specification, weight unquantisation).

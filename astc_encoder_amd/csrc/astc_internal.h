@@ -14,6 +14,7 @@ struct astc_ctx {
     int device = 0;
     astc::DeviceTables* d_tables = nullptr;  // device copy of the sRGB / trit tables
     astc::DeviceTables h_tables{};           // host copy
+    astc::TritQuintDecodeTable* d_dec_tables = nullptr;  // decoder tables
     std::mutex mu;                           // guards lazily created objects
 };
 
@@ -66,6 +67,9 @@ inline int mode_of(int block_dim, uint32_t flags) {
 
 hipError_t launch_downsample(const uint8_t* src, int w, int h, int bpp, uint8_t* dst, int ow, int oh,
                              hipStream_t stream);
+
+hipError_t launch_decode(const TritQuintDecodeTable* d_tables, const void* blocks, void* rgba, int width, int height,
+                         int bw, int bh, hipStream_t stream);
 
 // Plain per-texture launch (one lane per block).
 // bpl_override: blocks per lane (1, 2, 4 or 8); 0 chooses from the block count.

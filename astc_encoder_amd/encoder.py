@@ -117,6 +117,17 @@ class Encoder:
         return out
 
 
+    def decode(self, blocks, width, height, block_dim_x=4, block_dim_y=None, out=None, stream=None):
+        """Decode (nblocks, 16) device blocks to an (H, W, 4) uint8 device tensor."""
+        import torch
+        block_dim_y = block_dim_x if block_dim_y is None else block_dim_y
+        if out is None:
+            out = torch.empty((height, width, 4), dtype=torch.uint8, device=blocks.device)
+        check(lib().astc_decode(self._ctx, int(width), int(height), int(block_dim_x), int(block_dim_y),
+                                ctypes.c_void_p(blocks.contiguous().data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                _stream_handle(stream)))
+        return out
+
     def downsample(self, level, stream=None):
         """Next mip level of a device (H, W, C) uint8 tensor (2x2 box filter)."""
         import torch
@@ -327,6 +338,19 @@ def read_astc_header(path):
     check(lib().astc_read_file_header(str(path).encode(), ctypes.byref(bx), ctypes.byref(by),
                                       ctypes.byref(w), ctypes.byref(h), ctypes.byref(n)))
     return bx.value, by.value, w.value, h.value, n.value
+
+
+def load_tex(path, flip=True):
+    """load_tex's image step (main.cpp:19-56): a PNG as (H, W, 4) uint8 RGBA,
+    flipped vertically by default as the reference's loader does."""
+    w, h = ctypes.c_int(), ctypes.c_int()
+    buf = ctypes.POINTER(ctypes.c_uint8)()
+    check(lib().astc_load_image(str(path).encode(), int(bool(flip)), ctypes.byref(w), ctypes.byref(h),
+                                ctypes.byref(buf)))
+    try:
+        return np.ctypeslib.as_array(buf, shape=(h.value, w.value, 4)).copy()
+    finally:
+        lib().astc_free_image(buf)
 
 
 def srgb_table():

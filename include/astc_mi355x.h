@@ -81,6 +81,16 @@ int64_t astc_output_size(const astc_image_desc* desc);
  * One kernel launch on `stream`. */
 int astc_encode(astc_ctx* ctx, const astc_image_desc* desc, const void* src_dev, void* dst_dev, void* stream);
 
+/* Decode blocks (row-major, ceil(W/bx) x ceil(H/by) of them, 16-byte aligned)
+ * to a width x height RGBA8 image (4-byte aligned, row pitch width * 4).
+ * Single-partition LDR ASTC: block sizes 4..12, any weight grid and
+ * quantisation, colour endpoint modes 0/4/8/12, dual plane, LDR void extent;
+ * other blocks decode to magenta.  Texel = ((C0*(64-w) + C1*w + 32) >> 6) >> 8
+ * with C = endpoint * 257.  No reference counterpart (the reference only
+ * encodes); used to verify encoded textures on the device. */
+int astc_decode(astc_ctx* ctx, int width, int height, int block_dim_x, int block_dim_y, const void* blocks_dev,
+                void* rgba_dev, void* stream);
+
 /* The plain per-texture form over many images: astc_encode for each image in
  * turn (one kernel launch per image, issued from C++), on `stream`. */
 int astc_encode_each(astc_ctx* ctx, const astc_image_desc* descs, const void* const* srcs, void* const* dsts,
@@ -190,6 +200,14 @@ int astc_server_create(astc_ctx* ctx, const astc_server_config* config, astc_ser
 int astc_server_run_trace(astc_server* server, const astc_request* requests, int n, double* latency_s,
                           astc_server_stats* stats);
 int astc_server_destroy(astc_server* server);
+
+/* ---- image loading for the command-line drop-in ----------------------------
+ * Replaces load_tex's stbi_load(path, ..., STBI_rgb_alpha) with
+ * stbi_set_flip_vertically_on_load(1) (main.cpp:19-56) for PNG input: every PNG
+ * colour type and bit depth, expanded to 8-bit RGBA as stb_image does.  *rgba is
+ * width * height * 4 bytes, released with astc_free_image. */
+int astc_load_image(const char* path, int flip_vertically, int* width, int* height, uint8_t** rgba);
+void astc_free_image(uint8_t* rgba);
 
 /* Copy the library's sRGB table (256 floats: float(linear(c/255)) * 255.0f). */
 int astc_srgb_table(float out[256]);

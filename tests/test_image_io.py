@@ -1,0 +1,117 @@
+"""The PNG loader behind the command-line drop-in: every colour type and bit
+depth decodes to the RGBA that stb_image's STBI_rgb_alpha would give
+(main.cpp:25), checked against PIL decodes of the same files; vertical flip."""
+import struct
+import zlib
+
+import numpy as np
+import pytest
+from PIL import Image
+
+import astc_encoder_amd as astc
+
+
+def expected_rgba(img):
+    """stb_image's expansion: grey -> (g,g,g,255), grey+alpha -> (g,g,g,a), RGB -> a=255,
+    palette -> entries with tRNS alpha; 16-bit samples keep their high byte."""
+    if img.mode in ("I;16", "I;16B", "I"):
+        g = (np.array(img).astype(np.uint32) >> 8).astype(np.uint8)
+        return np.dstack([g, g, g, np.full_like(g, 255)])
+    return np.array(img.convert("RGBA"))
+
+
+@pytest.mark.parametrize("mode", ["L", "LA", "RGB", "RGBA", "P", "1", "I;16"])
+def test_png_colour_types(tmp_path, mode):
+    rng = np.random.default_rng(len(mode))
+    h, w = 37, 53
+    if mode == "P":
+        img = Image.fromarray(rng.integers(0, 256, (h, w, 3), dtype=np.uint8)).convert("P", palette=Image.ADAPTIVE)
+        img.info["transparency"] = 3
+    elif mode == "I;16":
+        img = Image.fromarray(rng.integers(0, 65536, (h, w), dtype=np.uint16))
+    elif mode == "1":
+        img = Image.fromarray(rng.integers(0, 2, (h, w), dtype=np.uint8) * 255).convert("1")
+    else:
+        ch = {"L": 1, "LA": 2, "RGB": 3, "RGBA": 4}[mode]
+        arr = rng.integers(0, 256, (h, w, ch), dtype=np.uint8)
+        img = Image.fromarray(arr[..., 0] if ch == 1 else arr, mode)
+    p = tmp_path / f"t_{mode.replace(';', '')}.png"
+    if mode == "P":
+        img.save(p, transparency=3)
+    else:
+        img.save(p)
+    got = astc.load_tex(p, flip=False)
+    want = expected_rgba(Image.open(p))
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(astc.load_tex(p), want[::-1])
+
+
+def write_png(path, rgba, interlace):
+    h, w, _ = rgba.shape
+
+    def chunk(tag, data):
+        return struct.pack(">I", len(data)) + tag + data + struct.pack(">I", zlib.crc32(tag + data) & 0xFFFFFFFF)
+
+    def scanlines(a, filt):
+        out = b""
+        prev = np.zeros(a.shape[1] * 4, np.int32)
+        for y in range(a.shape[0]):
+            row = a[y].reshape(-1).astype(np.int32)
+            if filt == 2:
+                data = ((row - prev) & 255).astype(np.uint8)
+            elif filt == 1:
+                left = np.concatenate([np.zeros(4, np.int32), row[:-4]])
+                data = ((row - left) & 255).astype(np.uint8)
+            else:
+                data = row.astype(np.uint8)
+            out += bytes([filt]) + data.tobytes()
+            prev = row
+        return out
+
+    raw = b""
+    if interlace:
+        for (x0, y0, dx, dy) in [(0, 0, 8, 8), (4, 0, 8, 8), (0, 4, 4, 8), (2, 0, 4, 4), (0, 2, 2, 4), (1, 0, 2, 2),
+                                 (0, 1, 1, 2)]:
+            sub = rgba[y0::dy, x0::dx]
+            if sub.size:
+                raw += scanlines(sub, 1 if (x0 + y0) % 2 else 2)
+    else:
+        raw = scanlines(rgba, 2)
+    png = b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 6, 0, 0, int(interlace)))
+    png += chunk(b"IDAT", zlib.compress(raw)) + chunk(b"IEND", b"")
+    path.write_bytes(png)
+
+
+@pytest.mark.parametrize("interlace", [False, True])
+def test_png_filters_and_adam7(tmp_path, interlace):
+    rng = np.random.default_rng(9)
+    rgba = rng.integers(0, 256, (23, 29, 4), dtype=np.uint8)
+    p = tmp_path / "x.png"
+    write_png(p, rgba, interlace)
+    np.testing.assert_array_equal(astc.load_tex(p, flip=False), rgba)
+    np.testing.assert_array_equal(np.array(Image.open(p).convert("RGBA")), rgba)
+
+
+def test_leaf_crop_fixture_loads(golden_dir):
+    import os
+    img = astc.load_tex(os.path.join(golden_dir, "leaf_crop.png"), flip=False)
+    np.testing.assert_array_equal(img, np.array(Image.open(os.path.join(golden_dir, "leaf_crop.png")).convert("RGBA")))
+
+
+def test_load_errors(tmp_path):
+    with pytest.raises(astc.AstcError):
+        astc.load_tex(tmp_path / "missing.png")
+    bad = tmp_path / "bad.png"
+    bad.write_bytes(b"not a png at all")
+    with pytest.raises(astc.AstcError):
+        astc.load_tex(bad)
+
+
+def test_cli_without_gpu_paths(tmp_path):
+    """Argument count and load failures behave as main.cpp:182-185, 215-218."""
+    import subprocess
+    r = subprocess.run([astc._capi.CLI_PATH], capture_output=True, text=True)
+    assert r.returncode == 255 and "wrong args count" in r.stdout
+    r = subprocess.run([astc._capi.CLI_PATH, str(tmp_path / "none.png"), "-alpha"], capture_output=True, text=True)
+    assert r.returncode == 255 and "load source texture failed!" in r.stdout
+    assert "has_alpha\ttrue" in r.stdout
