@@ -12,8 +12,11 @@
 //   * "x * s + t" expressions of the shader are single fused multiply-adds;
 //   * divisions by the constants N and N-1 are multiplications by the
 //     correctly rounded reciprocals (what FXC emits for a literal divisor);
-//   * normalize(v) = v * (1 / sqrtf(dot(v, v))) with correctly rounded sqrt and
-//     division; length(v) < 1e-5 is the equivalent test on dot(v, v).
+//   * normalize(v) = v * rsqrt_nr(dot(v, v)), a reciprocal square root built
+//     from IEEE operations only: seed 0x5f375a86 - (bits(x) >> 1), then three
+//     Newton steps y = y * fma(-(0.5x * y), y, 1.5) (within 4 ulp; the shader's
+//     own rsq is an approximate hardware instruction too);
+//   * length(v) < 1e-5 is the equivalent exact test on dot(v, v).
 // The file must be compiled with -ffp-contract=off so that nothing else fuses.
 #pragma once
 
@@ -41,8 +44,16 @@ __device__ __forceinline__ float dot4(const float a[4], const float b[4]) {
     return fma_(a[3], b[3], fma_(a[2], b[2], fma_(a[1], b[1], a[0] * b[0])));
 }
 
-// correctly rounded 1 / sqrt(x) as two IEEE operations
-__device__ __forceinline__ float rsqrt_cr(float x) { return 1.0f / __builtin_sqrtf(x); }
+// 1 / sqrt(x) from IEEE operations only (bit-pattern seed + three Newton steps),
+// reproduced bit for bit by the CPU oracle.
+__device__ __forceinline__ float rsqrt_nr(float x) {
+    float y = __uint_as_float(0x5f375a86u - (__float_as_uint(x) >> 1));
+    const float h = 0.5f * x;
+    y = y * fma_(-(h * y), y, 1.5f);
+    y = y * fma_(-(h * y), y, 1.5f);
+    y = y * fma_(-(h * y), y, 1.5f);
+    return y;
+}
 
 __device__ __forceinline__ bool is_short(float dot) { return dot < __uint_as_float(kLenThresholdBits); }
 
@@ -286,7 +297,7 @@ __device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], co
         float w[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) w[i] = dot4(m[i], u);
-        const float s = rsqrt_cr(dot4(w, w));
+        const float s = rsqrt_nr(dot4(w, w));
 #pragma unroll
         for (int i = 0; i < 4; ++i) v[i] = w[i] * s;
     }
@@ -344,7 +355,7 @@ __device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], co
 #pragma unroll
         for (int i = 0; i < 16; ++i) wq[i] = 0u;
     } else {
-        const float s = rsqrt_cr(kk);
+        const float s = rsqrt_nr(kk);
 #pragma unroll
         for (int c = 0; c < 4; ++c) k[c] = k[c] * s;
         float proj[16];

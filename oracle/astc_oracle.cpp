@@ -393,9 +393,30 @@ template <typename T> T clamp255(T x) { return std::fmin(std::fmax(x, T(0)), T(2
 template <typename T> bool short_vec(const Arith<T>& ar, const T* v) {
     return std::sqrt(ar.dot(v, v)) < K<T>::small;
 }
-/* normalize(v) = v * (1 / sqrt(dot(v, v))), correctly rounded sqrt and division. */
+/* Reciprocal square root of the canonical binary32 schedule: seed from the bit
+ * pattern, 0x5f375a86 - (bits(x) >> 1), then three Newton steps
+ * y = y * fma(-(0.5x * y), y, 1.5).  IEEE operations only, so every machine gets
+ * the same bits.  (HLSL's normalize is compiled to the GPU's approximate rsq.) */
+float rsqrt_nr(float x) {
+    uint32_t b;
+    std::memcpy(&b, &x, 4);
+    b = 0x5f375a86u - (b >> 1);
+    float y;
+    std::memcpy(&y, &b, 4);
+    const float h = 0.5f * x;
+    y = y * std::fma(-(h * y), y, 1.5f);
+    y = y * std::fma(-(h * y), y, 1.5f);
+    y = y * std::fma(-(h * y), y, 1.5f);
+    return y;
+}
+
+/* normalize(v) = v * rsqrt(dot(v, v)): the Newton rsqrt in the canonical binary32
+ * schedule, a correctly rounded 1 / sqrt otherwise (source schedule, FP64 twin). */
 template <typename T> void normalize4(const Arith<T>& ar, T* v) {
-    T s = T(1) / std::sqrt(ar.dot(v, v));
+    const T d = ar.dot(v, v);
+    T s;
+    if (std::is_same<T, float>::value && ar.canon) s = (T)rsqrt_nr((float)d);
+    else s = T(1) / std::sqrt(d);
     for (int c = 0; c < 4; ++c) v[c] = v[c] * s;
 }
 
@@ -937,6 +958,8 @@ double oracle_psnr(const uint8_t* src, int width, int height, int row_pitch, int
     if (sse <= 0.0) return INFINITY;
     return 10.0 * std::log10(255.0 * 255.0 * cnt / sse);
 }
+
+float oracle_rsqrt_nr(float x) { return rsqrt_nr(x); }
 
 float oracle_length_threshold(void) {
     /* Smallest x with sqrtf(x) >= SMALL_VALUE; sqrtf is monotone, so search near 1e-10. */

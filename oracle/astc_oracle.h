@@ -19,7 +19,8 @@
  * and fold divisions by constants):
  *   ORACLE_SCHED_CANONICAL  the documented schedule the HIP kernel implements:
  *                           row-wise fused multiply-add chains combined by a
- *                           fixed tree, reciprocal multiplication for 1/N.
+ *                           fixed tree, reciprocal multiplication for 1/N, a
+ *                           Newton reciprocal square root for normalize.
  *   ORACLE_SCHED_SOURCE     the HLSL source order taken literally: sequential
  *                           sums, separate multiply and add, IEEE division.
  * Both run in binary32 (float) or binary64 (double, the FP64 twin).
@@ -105,6 +106,9 @@ double oracle_psnr(const uint8_t* src, int width, int height, int row_pitch, int
 /* Squared error sum and sample count behind oracle_psnr (for aggregating over textures). */
 void oracle_sq_error(const uint8_t* src, int width, int height, int row_pitch, int src_format, int flags,
                      const uint8_t* decoded_rgba, double* sse, double* count);
+
+/* The canonical schedule's reciprocal square root (bit-pattern seed + three Newton steps). */
+float oracle_rsqrt_nr(float x);
 
 /* Smallest binary32 x with sqrtf(x) >= 1e-5f: "length(v) < SMALL_VALUE" <=> dot(v,v) < this. */
 float oracle_length_threshold(void);

@@ -55,6 +55,7 @@ def lib():
             "oracle_psnr": (ctypes.c_double, [P, I, I, I, I, I, P]),
             "oracle_sq_error": (None, [P, I, I, I, I, I, P, P, P]),
             "oracle_length_threshold": (ctypes.c_float, []),
+            "oracle_rsqrt_nr": (ctypes.c_float, [ctypes.c_float]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)

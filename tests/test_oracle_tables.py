@@ -207,3 +207,14 @@ def test_length_threshold_equivalence():
     xs = (np.arange(-200000, 200000, dtype=np.int64) + int(bitsv)).astype(np.uint32).view(np.float32)
     np.testing.assert_array_equal(np.sqrt(xs) < np.float32(1e-5), xs < thr)
     assert int(bitsv) == 0x2EDBE6FD
+
+
+def test_newton_rsqrt_accuracy():
+    """The canonical schedule's rsqrt stays within 4 ulp of 1/sqrt over the
+    range normalize sees (squared lengths from the 1e-10 threshold upward)."""
+    f = orc.lib().oracle_rsqrt_nr
+    xs = np.float32(10.0) ** np.linspace(-10, 12, 20001).astype(np.float32)
+    got = np.array([f(float(x)) for x in xs], np.float32)
+    exact = 1.0 / np.sqrt(xs.astype(np.float64))
+    ulp = np.spacing(exact.astype(np.float32)).astype(np.float64)
+    assert np.max(np.abs(got - exact) / ulp) <= 4.0
