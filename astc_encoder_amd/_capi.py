@@ -38,11 +38,11 @@ class ImageDesc(ctypes.Structure):
 class Tuning(ctypes.Structure):
     """astc_tuning"""
     _fields_ = [("blocks_per_lane", ctypes.c_int32), ("workgroups_per_cu", ctypes.c_int32),
-                ("prefetch", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("prefetch", ctypes.c_int32), ("split_modes", ctypes.c_int32)]
 
     def as_dict(self):
         return {"blocks_per_lane": self.blocks_per_lane, "workgroups_per_cu": self.workgroups_per_cu,
-                "prefetch": self.prefetch}
+                "prefetch": self.prefetch, "split_modes": self.split_modes}
 
 
 class ServerConfig(ctypes.Structure):

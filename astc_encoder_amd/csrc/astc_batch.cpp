@@ -16,14 +16,15 @@ struct astc_batch {
     std::vector<BatchLevel> levels;  // host copies
     std::vector<int> modes;
     int64_t blocks = 0, texels = 0;
-    astc_tuning tuning{4, 2, 1, 0};
+    astc_tuning tuning{4, 2, 1, 1};
     // device work list
     BatchLevel* d_levels = nullptr;
     BatchSeg* d_segs = nullptr;
     BatchItem* d_items = nullptr;
-    int* d_scalars = nullptr;  // [0] = item count, [1] = queue head
+    int* d_scalars = nullptr;  // [0] item count, [1] first 4x4 item, [2..3] queue heads
     int nitems = 0;
-    hipStream_t capture_stream = nullptr;
+    hipStream_t capture_stream = nullptr, side_stream = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -35,11 +36,16 @@ namespace astc {
 // into chunks of 256 x blocks_per_lane blocks, images under 256 blocks packed
 // up to kMaxSegs per item (one block per thread).
 void plan_work(const std::vector<BatchLevel>& levels, const std::vector<int>& modes, int blocks_per_lane,
-               std::vector<BatchSeg>* segs, std::vector<BatchItem>* items) {
+               std::vector<BatchSeg>* segs, std::vector<BatchItem>* items, int* first4) {
     segs->clear();
     items->clear();
+    *first4 = -1;
     const int chunk = kPlainThreads * blocks_per_lane;
-    for (int mode = MODE_6x6_RGBA; mode >= MODE_4x4_RGB; --mode) {
+    int order[MODE_COUNT];
+    for (int m = 0; m < MODE_COUNT; ++m) order[m] = m;
+    std::stable_sort(order, order + MODE_COUNT, [](int a, int b) { return mode_cost_rank(a) > mode_cost_rank(b); });
+    for (int mode : order) {
+        if (!(mode & 2) && *first4 < 0) *first4 = (int)items->size();  // first 4x4 mode
         std::vector<int> idx;
         for (int i = 0; i < (int)levels.size(); ++i)
             if (modes[i] == mode) idx.push_back(i);
@@ -75,6 +81,7 @@ void plan_work(const std::vector<BatchLevel>& levels, const std::vector<int>& mo
         }
         flush();
     }
+    if (*first4 < 0) *first4 = (int)items->size();
 }
 
 }  // namespace astc
@@ -96,7 +103,8 @@ int build(astc_batch* b) {
     free_worklist(b);
     std::vector<BatchSeg> segs;
     std::vector<BatchItem> items;
-    plan_work(b->levels, b->modes, b->tuning.blocks_per_lane, &segs, &items);
+    int first4 = 0;
+    plan_work(b->levels, b->modes, b->tuning.blocks_per_lane, &segs, &items, &first4);
     hipError_t e;
     if ((e = hipMalloc(&b->d_segs, std::max<size_t>(1, segs.size()) * sizeof(BatchSeg))) != hipSuccess)
         return hip_fail(e, "hipMalloc(segs)");
@@ -109,7 +117,7 @@ int build(astc_batch* b) {
                                          hipMemcpyHostToDevice)) != hipSuccess)
         return hip_fail(e, "hipMemcpy(items)");
     b->nitems = (int)items.size();
-    int scal[2] = {b->nitems, 0};
+    int scal[4] = {b->nitems, first4, 0, 0};
     if ((e = hipMemcpy(b->d_scalars, scal, sizeof(scal), hipMemcpyHostToDevice)) != hipSuccess)
         return hip_fail(e, "hipMemcpy(scalars)");
 
@@ -117,14 +125,20 @@ int build(astc_batch* b) {
     a.levels = b->d_levels;
     a.segs = b->d_segs;
     a.items = b->d_items;
-    a.nitems = b->d_scalars;
-    a.counter = b->d_scalars + 1;
+    a.scalars = b->d_scalars;
+    a.counters = b->d_scalars + 2;
     a.tables = b->ctx->d_tables;
     a.prefetch = b->tuning.prefetch;
-    const int grid = std::max(1, b->num_cus * b->tuning.workgroups_per_cu);
+    BatchLaunch l;
+    l.grid = std::max(1, b->num_cus * b->tuning.workgroups_per_cu);
+    l.grid4 = 2 * l.grid;
+    l.split = b->tuning.split_modes;
+    l.side = b->side_stream;
+    l.fork = b->fork;
+    l.join = b->join;
     if ((e = hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
         return hip_fail(e, "hipStreamBeginCapture");
-    hipError_t le = launch_encode_batch(a, grid, b->capture_stream);
+    hipError_t le = launch_encode_batch(a, l, b->capture_stream);
     e = hipStreamEndCapture(b->capture_stream, &b->graph);
     if (le != hipSuccess) return hip_fail(le, "batch launch capture");
     if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
@@ -136,7 +150,8 @@ int build(astc_batch* b) {
 bool valid_tuning(const astc_tuning* t) {
     const int bpl = t->blocks_per_lane;
     return (bpl == 1 || bpl == 2 || bpl == 4 || bpl == 8) && t->workgroups_per_cu >= 1 &&
-           t->workgroups_per_cu <= 4 && (t->prefetch == 0 || t->prefetch == 1);
+           t->workgroups_per_cu <= 4 && (t->prefetch == 0 || t->prefetch == 1) &&
+           (t->split_modes == 0 || t->split_modes == 1);
 }
 
 }  // namespace
@@ -173,10 +188,11 @@ int astc_batch_create(astc_ctx* ctx, const astc_image_desc* descs, const void* c
         L.f.fmt = descs[i].src_format;
         L.f.flags = descs[i].flags;
         L.f.aligned = ((reinterpret_cast<uintptr_t>(srcs[i]) & 3u) == 0) && ((descs[i].row_pitch & 3) == 0);
+        L.f.or_mask = normal_or_mask(descs[i].flags);
         L.dst = static_cast<uint32_t*>(dsts[i]);
         L.xblocks = g.xblocks;
         L.nblocks = (int)g.nblocks;
-        b->modes[i] = mode_of(descs[i].block_dim, descs[i].flags);
+        b->modes[i] = mode_of(descs[i].block_dim, descs[i].src_format, descs[i].flags, descs[i].width, descs[i].height);
         b->blocks += g.nblocks;
         b->texels += (int64_t)descs[i].width * descs[i].height;
     }
@@ -188,10 +204,13 @@ int astc_batch_create(astc_ctx* ctx, const astc_image_desc* descs, const void* c
     else if ((e = hipMemcpy(b->d_levels, b->levels.data(), count * sizeof(BatchLevel), hipMemcpyHostToDevice)) !=
              hipSuccess)
         rc = hip_fail(e, "hipMemcpy(levels)");
-    else if ((e = hipMalloc(&b->d_scalars, 2 * sizeof(int))) != hipSuccess)
+    else if ((e = hipMalloc(&b->d_scalars, 4 * sizeof(int))) != hipSuccess)
         rc = hip_fail(e, "hipMalloc(scalars)");
-    else if ((e = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking)) != hipSuccess)
-        rc = hip_fail(e, "hipStreamCreate");
+    else if ((e = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking)) != hipSuccess ||
+             (e = hipStreamCreateWithFlags(&b->side_stream, hipStreamNonBlocking)) != hipSuccess ||
+             (e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming)) != hipSuccess ||
+             (e = hipEventCreateWithFlags(&b->join, hipEventDisableTiming)) != hipSuccess)
+        rc = hip_fail(e, "stream/event creation");
     else
         rc = build(b);
     if (rc != ASTC_OK) {
@@ -208,7 +227,6 @@ int astc_batch_set_tuning(astc_batch* b, const astc_tuning* t) {
     hipError_t e = hipSetDevice(b->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     b->tuning = *t;
-    b->tuning.reserved = 0;
     return build(b);
 }
 
@@ -241,11 +259,12 @@ int astc_batch_autotune(astc_batch* b, void* stream, int reps, astc_tuning* best
     astc_tuning best = b->tuning;
     float best_ms = 1e30f;
     int rc = ASTC_OK;
+    for (int split : {0, 1})
     for (int bpl : {1, 2, 4, 8})
         for (int wpc : {1, 2})
             for (int pf : {0, 1}) {
                 if (bpl == 1 && pf == 1) continue;  // nothing to prefetch within an item
-                astc_tuning t{bpl, wpc, pf, 0};
+                astc_tuning t{bpl, wpc, pf, split};
                 if ((rc = astc_batch_set_tuning(b, &t)) != ASTC_OK) break;
                 if ((rc = astc_batch_launch(b, stream)) != ASTC_OK) break;  // warm-up
                 (void)hipEventRecord(t0, s);
@@ -285,6 +304,9 @@ int astc_batch_destroy(astc_batch* b) {
     if (b->d_levels) (void)hipFree(b->d_levels);
     if (b->d_scalars) (void)hipFree(b->d_scalars);
     if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
+    if (b->side_stream) (void)hipStreamDestroy(b->side_stream);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->join) (void)hipEventDestroy(b->join);
     delete b;
     return ASTC_OK;
 }

@@ -16,7 +16,12 @@
 //     from IEEE operations only: seed 0x5f375a86 - (bits(x) >> 1), then three
 //     Newton steps y = y * fma(-(0.5x * y), y, 1.5) (within 4 ulp; the shader's
 //     own rsq is an approximate hardware instruction too);
-//   * length(v) < 1e-5 is the equivalent exact test on dot(v, v).
+//   * length(v) < 1e-5 is the equivalent exact test on dot(v, v);
+//   * three exact rearrangements of the shader's formulas save a quarter of
+//     the arithmetic: the power iteration runs on S = M M with its single
+//     effective length test, min/max projections are dot(texel, v) minus
+//     dot(mean, v), and grid weights rescale texel projections (dot(k, e0)
+//     cancels in the rescaling).  Each is explained where it is used.
 // The file must be compiled with -ffp-contract=off so that nothing else fuses.
 #pragma once
 
@@ -44,6 +49,17 @@ __device__ __forceinline__ float dot4(const float a[4], const float b[4]) {
     return fma_(a[3], b[3], fma_(a[2], b[2], fma_(a[1], b[1], a[0] * b[0])));
 }
 
+// dot4 over the first NC channels.  When the other channels hold zero in one
+// operand (the products there are zeros) this is dot4's value: adding a zero
+// product changes at most the sign of a zero result.
+template <int NC>
+__device__ __forceinline__ float dotn(const float a[4], const float b[4]) {
+    float r = a[0] * b[0];
+#pragma unroll
+    for (int c = 1; c < NC; ++c) r = fma_(a[c], b[c], r);
+    return r;
+}
+
 // 1 / sqrt(x) from IEEE operations only (bit-pattern seed + three Newton steps),
 // reproduced bit for bit by the CPU oracle.
 __device__ __forceinline__ float rsqrt_nr(float x) {
@@ -55,18 +71,13 @@ __device__ __forceinline__ float rsqrt_nr(float x) {
     return y;
 }
 
+// frexp exponent: x = m * 2^e with 0.5 <= |m| < 1 (x finite, nonzero).
+__device__ __forceinline__ int exponent_of(float x) { return __builtin_amdgcn_frexp_expf(x); }
+__device__ __forceinline__ float ldexp_(float x, int e) { return __builtin_amdgcn_ldexpf(x, e); }
+
 __device__ __forceinline__ bool is_short(float dot) { return dot < __uint_as_float(kLenThresholdBits); }
 
 __device__ __forceinline__ float clamp255(float x) { return fminf(fmaxf(x, 0.0f), 255.0f); }
-
-// Returns x unchanged but hides its value from the optimiser.  Used so that the
-// compiler recomputes texel - mean in the min/max pass instead of keeping the
-// covariance pass's differences alive across the power iteration (which would
-// double the registers held per block).
-__device__ __forceinline__ float opaque(float x) {
-    asm volatile("" : "+v"(x));
-    return x;
-}
 
 // Row-partial combination tree.
 template <int DIM>
@@ -91,8 +102,15 @@ struct FetchArgs {
     const uint8_t* src;
     int width, height, row_pitch, fmt;
     uint32_t flags;
-    int aligned;  // src and row_pitch are multiples of 4
+    int aligned;       // src and row_pitch are multiples of 4
+    uint32_t or_mask;  // 0xFFFF0000 for normal maps (blue = alpha = 255), else 0
 };
+
+// OR-mask that applies the normal-map override to a packed RGBA8 texel.  Setting
+// the bytes to 255 before conversion equals forcing the floats to 255 after it
+// (ASTC_Encode.hlsl:575-579): the sRGB table maps 255 to exactly 255.0f, and
+// out-of-bounds (zero) texels get blue = alpha = 255 as well.
+__host__ __device__ inline uint32_t normal_or_mask(uint32_t flags) { return (flags & FLAG_NORMAL) ? 0xFFFF0000u : 0u; }
 
 __device__ __forceinline__ int bytes_per_texel(int fmt) { return fmt == FMT_RGBA8 ? 4 : fmt == FMT_RGB8 ? 3 : 2; }
 
@@ -138,21 +156,19 @@ __device__ __forceinline__ void load_row_fast(const FetchArgs& a, int x0, int y,
     }
 }
 
-__device__ __forceinline__ void unpack_texel(uint32_t p, uint32_t flags, const float* lut, float t[4]) {
-    if (flags & FLAG_SRGB) {
-        t[0] = lut[p & 0xFFu];
-        t[1] = lut[(p >> 8) & 0xFFu];
-        t[2] = lut[(p >> 16) & 0xFFu];
-    } else {
-        t[0] = float(p & 0xFFu);
-        t[1] = float((p >> 8) & 0xFFu);
-        t[2] = float((p >> 16) & 0xFFu);
-    }
-    t[3] = float(p >> 24);
-    if (flags & FLAG_NORMAL) {
-        t[2] = 255.0f;
-        t[3] = 255.0f;
-    }
+// RGBA8 -> floats (x255 scale); SRGB linearises the colour channels by table.
+// The normal-map override is already in the word (FetchArgs::or_mask).
+// Only the first NC channels are converted (the others are known constants to
+// the encoder, see encode_block).
+template <bool SRGB, int NC = 4>
+__device__ __forceinline__ void unpack_texel(uint32_t p, const float* lut, float t[4]) {
+#pragma unroll
+    for (int c = 0; c < NC && c < 3; ++c) t[c] = SRGB ? lut[(p >> (8 * c)) & 0xFFu] : float((p >> (8 * c)) & 0xFFu);
+    if (NC == 4) t[3] = float(p >> 24);
+    // Pin the converted floats: left to itself the compiler keeps the packed
+    // words alive next to them and re-converts, which costs registers.
+#pragma unroll
+    for (int c = 0; c < NC; ++c) asm volatile("" : "+v"(t[c]));
 }
 
 // Load a block's texels as packed RGBA8 words (loads only; no conversion), so
@@ -176,11 +192,11 @@ __device__ __forceinline__ void load_block(const FetchArgs& a, int bx, int by, u
     }
 }
 
-template <int DIM>
-__device__ __forceinline__ void unpack_block(const uint32_t p[DIM * DIM], uint32_t flags, const float* lut,
+template <int DIM, bool SRGB, int NC = 4>
+__device__ __forceinline__ void unpack_block(const uint32_t p[DIM * DIM], uint32_t or_mask, const float* lut,
                                              float x[DIM * DIM][4]) {
 #pragma unroll
-    for (int k = 0; k < DIM * DIM; ++k) unpack_texel(p[k], flags, lut, x[k]);
+    for (int k = 0; k < DIM * DIM; ++k) unpack_texel<SRGB, NC>(p[k] | or_mask, lut, x[k]);
 }
 
 // ---------------------------------------------------------------------------
@@ -234,17 +250,26 @@ __device__ __forceinline__ Block128 pack_block(const uint32_t e0q[4], const uint
 
 // ---------------------------------------------------------------------------
 // Floating-point stage (ASTC_Encode.hlsl:93-168, 233-245, 256-393, 510-550)
+//
+// NC < 4: channels NC..3 hold exactly 255 in every texel of the block (normal
+// maps: blue and alpha; an RGB8 block inside the image: alpha).  Such a channel
+// has mean exactly 255 (16 and 36 copies of an integer sum and scale back
+// exactly), so its differences from the mean, its covariance row, its axis
+// component after the first product and its endpoint offsets are all zero, and
+// the four-channel computation adds only zero terms for it.  The specialised
+// code leaves those terms out; it produces the same bytes.
 // ---------------------------------------------------------------------------
-template <int DIM, bool ALPHA>
+template <int DIM, bool ALPHA, int NC = 4>
 __device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], const uint8_t* trit_lds) {
+    static_assert(NC >= 2 && NC <= 4, "two to four active channels");
     constexpr int N = DIM * DIM;
     constexpr float rcp_n = 1.0f / float(N);
     constexpr float rcp_n1 = 1.0f / float(N - 1);
 
     // mean (principal_component_analysis :142-147)
-    float mean[4];
+    float mean[4] = {255.0f, 255.0f, 255.0f, 255.0f};
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < NC; ++c) {
         RowTree<DIM> tree;
 #pragma unroll
         for (int y = 0; y < DIM; ++y) {
@@ -256,68 +281,98 @@ __device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], co
         mean[c] = tree.total() * rcp_n;
     }
 
-    // covariance (:149-162), upper triangle 00 01 02 03 11 12 13 22 23 33
-    float cov[10];
+    // covariance (:149-162): upper triangle of the active channels
+    constexpr int NP = NC * (NC + 1) / 2;
+    float cov[NP];
     {
-        RowTree<DIM> tree[10];
+        RowTree<DIM> tree[NP];
 #pragma unroll
         for (int y = 0; y < DIM; ++y) {
-            float r[10];
+            float r[NP];
 #pragma unroll
             for (int i = 0; i < DIM; ++i) {
                 float d[4];
 #pragma unroll
-                for (int c = 0; c < 4; ++c) d[c] = x[y * DIM + i][c] - mean[c];
+                for (int c = 0; c < NC; ++c) d[c] = x[y * DIM + i][c] - mean[c];
                 int p = 0;
 #pragma unroll
-                for (int a = 0; a < 4; ++a)
+                for (int a = 0; a < NC; ++a)
 #pragma unroll
-                    for (int b2 = a; b2 < 4; ++b2, ++p) r[p] = (i == 0) ? d[a] * d[b2] : fma_(d[a], d[b2], r[p]);
+                    for (int b2 = a; b2 < NC; ++b2, ++p) r[p] = (i == 0) ? d[a] * d[b2] : fma_(d[a], d[b2], r[p]);
             }
 #pragma unroll
-            for (int p = 0; p < 10; ++p) tree[p].add_row(y, r[p]);
+            for (int p = 0; p < NP; ++p) tree[p].add_row(y, r[p]);
         }
 #pragma unroll
-        for (int p = 0; p < 10; ++p) cov[p] = tree[p].total() * rcp_n1;
+        for (int p = 0; p < NP; ++p) cov[p] = tree[p].total() * rcp_n1;
     }
-    const float m[4][4] = {{cov[0], cov[1], cov[2], cov[3]},
-                           {cov[1], cov[4], cov[5], cov[6]},
-                           {cov[2], cov[5], cov[7], cov[8]},
-                           {cov[3], cov[6], cov[8], cov[9]}};
+    float m[4][4];
+    {
+        int p = 0;
+#pragma unroll
+        for (int a = 0; a < NC; ++a)
+#pragma unroll
+            for (int b2 = a; b2 < NC; ++b2, ++p) m[a][b2] = m[b2][a] = cov[p];
+    }
 
-    // eigen_vector (:93-106): v = M v; if length(v) < eps return v; v = normalize(M v)
+    // eigen_vector (:93-106).  One iteration is v <- normalize(M (M v)), after
+    // the test length(M v) < eps, which returns M v.  With S = M M formed once
+    // (S_ij = dot(row i, row j)) an iteration is v <- normalize(S v), and
+    // |M v|^2 = dot(v, S v) is the Rayleigh quotient of S at the unit v.  S is
+    // positive semi-definite, so that quotient never decreases from one power
+    // iterate to the next: only the test on the starting vector can succeed.
+    // Between iterations v is rescaled by a power of two (exact, and the
+    // direction is all the next product needs); the last step normalises.
+    float s2[4][4];
+#pragma unroll
+    for (int i = 0; i < NC; ++i)
+#pragma unroll
+        for (int j = i; j < NC; ++j) s2[i][j] = s2[j][i] = dotn<NC>(m[i], m[j]);
     float v[4] = {0.26726f, 0.80178f, 0.53452f, 0.0f};
-    for (int it = 0; it < 8; ++it) {
+    float w[4];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) w[i] = dotn<NC>(s2[i], v);
+    if (is_short(dotn<NC>(v, w))) {
         float u[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) u[i] = dot4(m[i], v);
+        for (int i = 0; i < NC; ++i) u[i] = dotn<NC>(m[i], v);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) v[i] = u[i];
-        if (is_short(dot4(u, u))) break;
-        float w[4];
+        for (int i = 0; i < NC; ++i) v[i] = u[i];
+    } else {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) w[i] = dot4(m[i], u);
-        const float s = rsqrt_nr(dot4(w, w));
+        for (int it = 1; it < 8; ++it) {
+            float mx = fabsf(w[0]);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) v[i] = w[i] * s;
+            for (int i = 1; i < NC; ++i) mx = fmaxf(mx, fabsf(w[i]));
+            const int e = exponent_of(mx);
+#pragma unroll
+            for (int i = 0; i < NC; ++i) v[i] = ldexp_(w[i], -e);
+#pragma unroll
+            for (int i = 0; i < NC; ++i) w[i] = dotn<NC>(s2[i], v);
+        }
+        const float sc = rsqrt_nr(dotn<NC>(w, w));
+#pragma unroll
+        for (int i = 0; i < NC; ++i) v[i] = w[i] * sc;
     }
-
-    // find_min_max (:108-137)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) mean[c] = opaque(mean[c]);
+    for (int i = NC; i < 4; ++i) v[i] = 0.0f;
+
+    // find_min_max (:108-137): t_k = dot(texel_k - mean, v), taken as
+    // dot(texel_k, v) - dot(mean, v); rounding is monotonic, so the minimum and
+    // maximum of t_k are the extreme projections minus dot(mean, v).
     float lo = 1e31f, hi = -1e31f;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        float d[4];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) d[c] = x[k][c] - mean[c];
-        const float t = dot4(d, v);
+        const float t = dotn<NC>(x[k], v);
         lo = fminf(lo, t);
         hi = fmaxf(hi, t);
     }
-    float e0[4], e1[4];
+    const float mv = dotn<NC>(mean, v);
+    lo = lo - mv;
+    hi = hi - mv;
+    float e0[4] = {255.0f, 255.0f, 255.0f, 255.0f}, e1[4] = {255.0f, 255.0f, 255.0f, 255.0f};
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < NC; ++c) {
         e0[c] = clamp255(fma_(v[c], lo, mean[c]));
         e1[c] = clamp255(fma_(v[c], hi, mean[c]));
     }
@@ -325,7 +380,7 @@ __device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], co
     const float s1 = (rintf(e1[0]) + rintf(e1[1])) + rintf(e1[2]);
     if (s0 > s1) {
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
+        for (int c = 0; c < NC; ++c) {
             const float t = e0[c];
             e0[c] = e1[c];
             e1[c] = t;
@@ -344,50 +399,55 @@ __device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], co
         e1q[c] = uint32_t(rintf(e1[c]));
     }
 
-    // calculate_normal_weights (:316-372) + quantize_weights (:374-382)
+    // calculate_normal_weights (:316-372) + quantize_weights (:374-382).  The
+    // weight of grid point i is dot(k, sample_i - e0) rescaled to [0, 1] by its
+    // minimum and range over the grid, so the constant dot(k, e0) cancels: the
+    // projections p_i = dot(k, sample_i) are rescaled instead.  A 6x6 sample is
+    // a fixed 4-tap combination of texels, and so is its projection.
     constexpr uint32_t range = ALPHA ? 5u : 11u;
+    constexpr int KC = (NC == 4 && !ALPHA) ? 3 : NC;  // RGB mode: endpoint alphas are equal
     uint32_t wq[16];
-    float k[4];
+    float k[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int c = 0; c < 4; ++c) k[c] = e1[c] - e0[c];
-    const float kk = dot4(k, k);
+    for (int c = 0; c < KC; ++c) k[c] = e1[c] - e0[c];
+    const float kk = dotn<KC>(k, k);
     if (is_short(kk)) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) wq[i] = 0u;
     } else {
         const float s = rsqrt_nr(kk);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) k[c] = k[c] * s;
+        for (int c = 0; c < KC; ++c) k[c] = k[c] * s;
         float proj[16];
+        if (DIM == 6) {
+            float pt[N];
+#pragma unroll
+            for (int j = 0; j < N; ++j) pt[j] = dotn<KC>(k, x[j]);
+            constexpr float tap_w[3] = {0.444f, 0.222f, 0.111f};
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                float smp = pt[grid6_tap_index(i, 0)] * tap_w[grid6_tap_far(i, 0)];
+                smp = fma_(pt[grid6_tap_index(i, 1)], tap_w[grid6_tap_far(i, 1)], smp);
+                smp = fma_(pt[grid6_tap_index(i, 2)], tap_w[grid6_tap_far(i, 2)], smp);
+                proj[i] = fma_(pt[grid6_tap_index(i, 3)], tap_w[grid6_tap_far(i, 3)], smp);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) proj[i] = dotn<KC>(k, x[i]);
+        }
         float minw = 1e31f, maxw = -1e31f;
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
-            float diff[4];
-            if (DIM == 6) {
-                constexpr float tap_w[3] = {0.444f, 0.222f, 0.111f};
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    float smp = x[grid6_tap_index(i, 0)][c] * tap_w[grid6_tap_far(i, 0)];
-                    smp = fma_(x[grid6_tap_index(i, 1)][c], tap_w[grid6_tap_far(i, 1)], smp);
-                    smp = fma_(x[grid6_tap_index(i, 2)][c], tap_w[grid6_tap_far(i, 2)], smp);
-                    smp = fma_(x[grid6_tap_index(i, 3)][c], tap_w[grid6_tap_far(i, 3)], smp);
-                    diff[c] = smp - e0[c];
-                }
-            } else {
-#pragma unroll
-                for (int c = 0; c < 4; ++c) diff[c] = x[i][c] - e0[c];
-            }
-            const float w = dot4(k, diff);
-            minw = fminf(w, minw);
-            maxw = fmaxf(w, maxw);
-            proj[i] = w;
+            minw = fminf(proj[i], minw);
+            maxw = fmaxf(proj[i], maxw);
         }
+        // (w - min) * invlen is at most 1 + 2^-22, so round(w * range) never
+        // exceeds range and the clamp of quantize_weight (:258) never acts.
         const float invlen = 1.0f / fmaxf(0.00001f, maxw - minw);
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             const float wn = (proj[i] - minw) * invlen;
-            const uint32_t q = uint32_t(rintf(wn * float(range)));
-            wq[i] = q < range ? q : range;
+            wq[i] = uint32_t(rintf(wn * float(range)));
         }
     }
     return pack_block<ALPHA>(e0q, e1q, wq, trit_lds);

@@ -40,7 +40,13 @@ void build_tables(DeviceTables* t);
 constexpr int kPlainThreads = 256;  // threads per workgroup of the encode kernels
 
 // ---- batched persistent encoder: device work list ----
-enum : int { MODE_4x4_RGB = 0, MODE_4x4_RGBA = 1, MODE_6x6_RGB = 2, MODE_6x6_RGBA = 3 };
+// work-item mode: block size x alpha, the sRGB fetch bit and the channel class
+// (which channels are 255 in every block: alpha of an RGB8 image tiled exactly
+// by its blocks, blue and alpha of a normal map)
+enum : int {
+    MODE_4x4_RGB = 0, MODE_4x4_RGBA = 1, MODE_6x6_RGB = 2, MODE_6x6_RGBA = 3, MODE_SRGB = 4,
+    MODE_CH_ALL = 0, MODE_CH_RGB8 = 8, MODE_CH_NORMAL = 16, MODE_CH_MASK = 24, MODE_COUNT = 32
+};
 constexpr int kMaxSegs = 16;  // images packed into one work item
 
 struct BatchLevel {   // one image (texture or mip level)
@@ -54,15 +60,32 @@ struct BatchArgs {
     const BatchLevel* levels;
     const BatchSeg* segs;
     const BatchItem* items;
-    const int* nitems;  // read on the device, so a captured graph can be re-fed
-    int* counter;       // work-queue head, reset by a kernel node before each run
+    const int* scalars;  // [0] item count, [1] index of the first 4x4 item; read on
+                         // the device, so a captured graph can be re-fed
+    int* counters;       // work-queue heads [all or 6x6, 4x4], reset by a kernel node
     const DeviceTables* tables;
     int prefetch;
 };
-// Launches the counter reset and the persistent kernel (grid workgroups).
-hipError_t launch_encode_batch(const BatchArgs& a, int grid, hipStream_t stream);
-inline int mode_of(int block_dim, uint32_t flags) {
-    return (block_dim == 6 ? 2 : 0) + ((flags & FLAG_ALPHA) ? 1 : 0);
+struct BatchLaunch {
+    int grid;      // workgroups of the (6x6 or only) persistent kernel
+    int grid4;     // workgroups of the 4x4 kernel when split
+    int split;     // 0: one kernel for every item; 1: 6x6 and 4x4 kernels
+    hipStream_t side;            // second stream for the 4x4 kernel (split only)
+    hipEvent_t fork, join;       // ordering events between the two streams
+};
+// Launches the counter reset and the persistent kernel(s) on `stream`.
+hipError_t launch_encode_batch(const BatchArgs& a, const BatchLaunch& l, hipStream_t stream);
+inline int mode_of(int block_dim, int src_format, uint32_t flags, int width, int height) {
+    const bool tiled = width % block_dim == 0 && height % block_dim == 0;
+    const int ch = (flags & FLAG_NORMAL) ? MODE_CH_NORMAL
+                   : (src_format == FMT_RGB8 && tiled) ? MODE_CH_RGB8 : MODE_CH_ALL;
+    return (block_dim == 6 ? 2 : 0) + ((flags & FLAG_ALPHA) ? 1 : 0) + ((flags & FLAG_SRGB) ? MODE_SRGB : 0) + ch;
+}
+// relative cost order of modes (most expensive first) for the work list: block
+// size, then active channels, alpha and sRGB
+inline int mode_cost_rank(int mode) {
+    const int ch = (mode & MODE_CH_MASK) == MODE_CH_NORMAL ? 0 : (mode & MODE_CH_MASK) == MODE_CH_RGB8 ? 1 : 2;
+    return ((mode & 2) ? 16 : 0) + ch * 4 + (mode & 1) * 2 + ((mode & MODE_SRGB) ? 1 : 0);
 }
 
 hipError_t launch_downsample(const uint8_t* src, int w, int h, int bpp, uint8_t* dst, int ow, int oh,

@@ -24,10 +24,13 @@ struct PlainArgs {
     const DeviceTables* tables;
 };
 
+// LDS tables: lut[0..255] sRGB -> float(linear) * 255, lut[256..511] the
+// identity (c -> float(c), the exact linear fetch), trit codes.
 __device__ __forceinline__ void stage_tables(const DeviceTables* t, float* lut, uint8_t* trit) {
     const int tid = threadIdx.x;
     for (int i = tid; i < 256; i += blockDim.x) {
         lut[i] = t->srgb[i];
+        lut[256 + i] = float(i);
         trit[i] = t->trit[i];
     }
 }
@@ -38,6 +41,29 @@ __device__ __forceinline__ void load_block_id(const FetchArgs& f, int xblocks, i
     load_block<DIM>(f, id - by * xblocks, by, p);
 }
 
+// Channel classes: which channels are 255 in every texel of every block.
+//   CH_ALL    any content: four channels;
+//   CH_RGB8   an RGB8 image tiled exactly by its blocks (no texel outside the
+//             image, where alpha would be 0): alpha is constant, three channels;
+//   CH_NORMAL normal maps (blue = alpha = 255, outside the image too): two.
+enum : int { CH_ALL = 4, CH_RGB8 = 3, CH_NORMAL = 2 };
+
+__host__ __device__ inline int channel_class(int fmt, uint32_t flags, int width, int height, int dim) {
+    if (flags & FLAG_NORMAL) return CH_NORMAL;
+    return (fmt == FMT_RGB8 && width % dim == 0 && height % dim == 0) ? CH_RGB8 : CH_ALL;
+}
+
+// SRGB: 1 / 0 fixes the fetch at compile time, -1 selects the table per image
+// (sRGB or identity, no branch).  NC: the channel class's active channels.
+template <int DIM, bool ALPHA, int SRGB, int NC>
+__device__ __forceinline__ Block128 encode_texels(const uint32_t p[DIM * DIM], const FetchArgs& f, const float* lut,
+                                                  const uint8_t* trit) {
+    float x[DIM * DIM][4];
+    if (SRGB < 0) unpack_block<DIM, true, NC>(p, f.or_mask, lut + ((f.flags & FLAG_SRGB) ? 0 : 256), x);
+    else unpack_block<DIM, SRGB == 1, NC>(p, f.or_mask, lut, x);
+    return encode_block<DIM, ALPHA, NC>(x, trit);
+}
+
 __device__ __forceinline__ void store_block(uint32_t* dst, int id, const Block128& b) {
     reinterpret_cast<uint4*>(dst)[id] = make_uint4(b.w[0], b.w[1], b.w[2], b.w[3]);
 }
@@ -46,7 +72,7 @@ __device__ __forceinline__ void store_block(uint32_t* dst, int id, const Block12
 // threads: step k, thread t takes block first + k * blockDim.x + t.  With
 // prefetch the next step's texels are loaded before the current block is
 // encoded.
-template <int DIM, bool ALPHA>
+template <int DIM, bool ALPHA, int SRGB, int CH>
 __device__ __forceinline__ void encode_chunk(const FetchArgs& f, int xblocks, uint32_t* dst, int first, int count,
                                              bool prefetch, const float* lut, const uint8_t* trit) {
     const int stride = blockDim.x;
@@ -59,9 +85,7 @@ __device__ __forceinline__ void encode_chunk(const FetchArgs& f, int xblocks, ui
         const bool has_next = o + stride < count;
         uint32_t nxt[DIM * DIM];
         if (has_next && prefetch) load_block_id<DIM>(f, xblocks, first + o + stride, nxt);
-        float x[DIM * DIM][4];
-        unpack_block<DIM>(cur, f.flags, lut, x);
-        store_block(dst, first + o, encode_block<DIM, ALPHA>(x, trit));
+        store_block(dst, first + o, encode_texels<DIM, ALPHA, SRGB, CH>(cur, f, lut, trit));
         if (!has_next) break;
         if (prefetch) {
 #pragma unroll
@@ -72,16 +96,22 @@ __device__ __forceinline__ void encode_chunk(const FetchArgs& f, int xblocks, ui
     }
 }
 
-template <int DIM, bool ALPHA>
-__global__ void __launch_bounds__(kPlainThreads) encode_plain_kernel(PlainArgs a, int bpl) {
-    __shared__ float lut[256];
+// 4x4 blocks fit in 128 VGPRs (four waves per SIMD); 6x6 blocks hold 144
+// texel floats and run at two.
+template <int DIM>
+struct PlainWaves { static constexpr int value = DIM == 4 ? 4 : 2; };
+
+template <int DIM, bool ALPHA, bool SRGB, int CH>
+__global__ void __launch_bounds__(kPlainThreads)
+__attribute__((amdgpu_waves_per_eu(PlainWaves<DIM>::value))) encode_plain_kernel(PlainArgs a, int bpl) {
+    __shared__ float lut[512];
     __shared__ uint8_t trit[256];
     stage_tables(a.tables, lut, trit);
     __syncthreads();
     const int chunk = kPlainThreads * bpl;
     const int first = blockIdx.x * chunk;
     const int count = min(chunk, a.nblocks - first);
-    encode_chunk<DIM, ALPHA>(a.f, a.xblocks, a.dst, first, count, true, lut, trit);
+    encode_chunk<DIM, ALPHA, SRGB ? 1 : 0, CH>(a.f, a.xblocks, a.dst, first, count, true, lut, trit);
 }
 
 // ---------------------------------------------------------------------------
@@ -91,17 +121,17 @@ __global__ void __launch_bounds__(kPlainThreads) encode_plain_kernel(PlainArgs a
 // one block per thread).  Items are uniform in mode (block size x alpha), so
 // each workgroup runs one encoder specialisation per item.
 // ---------------------------------------------------------------------------
-__global__ void reset_counter_kernel(int* counter) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
+__global__ void reset_counter_kernel(int* counters) {
+    if (threadIdx.x < 2 && blockIdx.x == 0) counters[threadIdx.x] = 0;
 }
 
-template <int DIM, bool ALPHA>
+template <int DIM, bool ALPHA, int CH>
 __device__ __forceinline__ void process_item(const BatchArgs& a, const BatchItem& item, const float* lut,
                                              const uint8_t* trit) {
     if (item.nseg == 1) {
         const BatchSeg sg = a.segs[item.seg_begin];
         const BatchLevel& L = a.levels[sg.level];
-        encode_chunk<DIM, ALPHA>(L.f, L.xblocks, L.dst, sg.first_block, sg.count, a.prefetch != 0, lut, trit);
+        encode_chunk<DIM, ALPHA, -1, CH>(L.f, L.xblocks, L.dst, sg.first_block, sg.count, a.prefetch != 0, lut, trit);
         return;
     }
     int o = threadIdx.x;
@@ -111,41 +141,84 @@ __device__ __forceinline__ void process_item(const BatchArgs& a, const BatchItem
             const BatchLevel& L = a.levels[sg.level];
             uint32_t p[DIM * DIM];
             load_block_id<DIM>(L.f, L.xblocks, sg.first_block + o, p);
-            float x[DIM * DIM][4];
-            unpack_block<DIM>(p, L.f.flags, lut, x);
-            store_block(L.dst, sg.first_block + o, encode_block<DIM, ALPHA>(x, trit));
+            store_block(L.dst, sg.first_block + o, encode_texels<DIM, ALPHA, -1, CH>(p, L.f, lut, trit));
             return;
         }
         o -= sg.count;
     }
 }
 
-__global__ void __launch_bounds__(kPlainThreads) encode_batch_kernel(BatchArgs a) {
-    __shared__ float lut[256];
+template <int DIM, bool ALPHA>
+__device__ __forceinline__ void process_item_ch(const BatchArgs& a, const BatchItem& item, const float* lut,
+                                                const uint8_t* trit) {
+    switch (item.mode & MODE_CH_MASK) {
+    case MODE_CH_NORMAL: process_item<DIM, ALPHA, CH_NORMAL>(a, item, lut, trit); break;
+    case MODE_CH_RGB8: process_item<DIM, ALPHA, CH_RGB8>(a, item, lut, trit); break;
+    default: process_item<DIM, ALPHA, CH_ALL>(a, item, lut, trit); break;
+    }
+}
+
+// SEL = 0: every item (one kernel holds all four block-size x alpha paths and
+// the 6x6 register budget, two waves per SIMD); SEL = 6 / 4: only the 6x6 items
+// (the front of the list) or only the 4x4 items, each with its own register
+// budget (the 4x4 kernel runs at four waves per SIMD).
+template <int SEL>
+struct BatchWaves { static constexpr int value = SEL == 4 ? 3 : 2; };
+
+template <int SEL>
+__global__ void __launch_bounds__(kPlainThreads) __attribute__((amdgpu_waves_per_eu(BatchWaves<SEL>::value)))
+encode_batch_kernel(BatchArgs a) {
+    __shared__ float lut[512];
     __shared__ uint8_t trit[256];
     __shared__ int next_item;
     stage_tables(a.tables, lut, trit);
-    const int nitems = *a.nitems;
+    const int nitems = a.scalars[0], first4 = a.scalars[1];
+    const int begin = SEL == 4 ? first4 : 0;
+    const int end = SEL == 6 ? first4 : nitems;
+    int* counter = a.counters + (SEL == 4 ? 1 : 0);
     for (;;) {
         __syncthreads();  // tables staged / everyone done with the previous item index
-        if (threadIdx.x == 0) next_item = atomicAdd(a.counter, 1);
+        if (threadIdx.x == 0) next_item = begin + atomicAdd(counter, 1);
         __syncthreads();
         const int it = __builtin_amdgcn_readfirstlane(next_item);
-        if (it >= nitems) break;
+        if (it >= end) break;
         const BatchItem item = a.items[it];
-        switch (item.mode) {
-        case MODE_4x4_RGB: process_item<4, false>(a, item, lut, trit); break;
-        case MODE_4x4_RGBA: process_item<4, true>(a, item, lut, trit); break;
-        case MODE_6x6_RGB: process_item<6, false>(a, item, lut, trit); break;
-        default: process_item<6, true>(a, item, lut, trit); break;
+        // one code path per block size x alpha; linear and sRGB fetches share
+        // it through the table choice (separate specialisations would not fit
+        // the register budget)
+        switch (item.mode & 3) {
+        case MODE_4x4_RGB:
+            if (SEL != 6) process_item_ch<4, false>(a, item, lut, trit);
+            break;
+        case MODE_4x4_RGBA:
+            if (SEL != 6) process_item_ch<4, true>(a, item, lut, trit);
+            break;
+        case MODE_6x6_RGB:
+            if (SEL != 4) process_item_ch<6, false>(a, item, lut, trit);
+            break;
+        default:
+            if (SEL != 4) process_item_ch<6, true>(a, item, lut, trit);
+            break;
         }
     }
 }
 
-hipError_t launch_encode_batch(const BatchArgs& a, int grid, hipStream_t stream) {
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, a.counter);
-    hipLaunchKernelGGL(encode_batch_kernel, dim3(unsigned(grid)), dim3(kPlainThreads), 0, stream, a);
-    return hipGetLastError();
+hipError_t launch_encode_batch(const BatchArgs& a, const BatchLaunch& l, hipStream_t stream) {
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, a.counters);
+    if (!l.split) {
+        hipLaunchKernelGGL(encode_batch_kernel<0>, dim3(unsigned(l.grid)), dim3(kPlainThreads), 0, stream, a);
+        return hipGetLastError();
+    }
+    // 6x6 kernel first on `stream`; the 4x4 kernel on the side stream picks up
+    // compute units as 6x6 workgroups finish; joined back into `stream`.
+    hipError_t e;
+    if ((e = hipEventRecord(l.fork, stream)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(l.side, l.fork, 0)) != hipSuccess) return e;
+    hipLaunchKernelGGL(encode_batch_kernel<6>, dim3(unsigned(l.grid)), dim3(kPlainThreads), 0, stream, a);
+    hipLaunchKernelGGL(encode_batch_kernel<4>, dim3(unsigned(l.grid4)), dim3(kPlainThreads), 0, l.side, a);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = hipEventRecord(l.join, l.side)) != hipSuccess) return e;
+    return hipStreamWaitEvent(stream, l.join, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -182,10 +255,22 @@ hipError_t launch_downsample(const uint8_t* src, int w, int h, int bpp, uint8_t*
     return hipGetLastError();
 }
 
+template <int DIM, bool ALPHA, int CH>
+static void launch_plain_ch(const PlainArgs& a, const dim3 grid, int bpl, hipStream_t stream) {
+    if (a.f.flags & FLAG_SRGB)
+        hipLaunchKernelGGL((encode_plain_kernel<DIM, ALPHA, true, CH>), grid, dim3(kPlainThreads), 0, stream, a, bpl);
+    else
+        hipLaunchKernelGGL((encode_plain_kernel<DIM, ALPHA, false, CH>), grid, dim3(kPlainThreads), 0, stream, a, bpl);
+}
+
 template <int DIM, bool ALPHA>
 static void launch_plain_t(const PlainArgs& a, int64_t nblocks, int bpl, hipStream_t stream) {
-    const dim3 grid(unsigned((nblocks + kPlainThreads * bpl - 1) / (kPlainThreads * bpl))), block(kPlainThreads);
-    hipLaunchKernelGGL((encode_plain_kernel<DIM, ALPHA>), grid, block, 0, stream, a, bpl);
+    const dim3 grid(unsigned((nblocks + kPlainThreads * bpl - 1) / (kPlainThreads * bpl)));
+    switch (channel_class(a.f.fmt, a.f.flags, a.f.width, a.f.height, DIM)) {
+    case CH_NORMAL: launch_plain_ch<DIM, ALPHA, CH_NORMAL>(a, grid, bpl, stream); break;
+    case CH_RGB8: launch_plain_ch<DIM, ALPHA, CH_RGB8>(a, grid, bpl, stream); break;
+    default: launch_plain_ch<DIM, ALPHA, CH_ALL>(a, grid, bpl, stream); break;
+    }
 }
 
 // Blocks per lane for a plain launch: enough workgroups to give every SIMD of
@@ -209,6 +294,7 @@ hipError_t launch_encode_plain(const DeviceTables* d_tables, const astc_image_de
     a.f.fmt = d.src_format;
     a.f.flags = d.flags;
     a.f.aligned = ((reinterpret_cast<uintptr_t>(src) & 3u) == 0) && ((d.row_pitch & 3) == 0);
+    a.f.or_mask = normal_or_mask(d.flags);
     a.xblocks = g.xblocks;
     a.nblocks = int(g.nblocks);
     a.dst = static_cast<uint32_t*>(dst);

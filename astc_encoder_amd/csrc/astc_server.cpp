@@ -24,7 +24,7 @@ using namespace astc;
 
 namespace astc {
 void plan_work(const std::vector<BatchLevel>& levels, const std::vector<int>& modes, int blocks_per_lane,
-               std::vector<BatchSeg>* segs, std::vector<BatchItem>* items);
+               std::vector<BatchSeg>* segs, std::vector<BatchItem>* items, int* first4);
 }
 
 namespace {
@@ -82,12 +82,12 @@ class Ring {
 };
 
 struct Slot {
-    hipStream_t stream = nullptr;
-    hipEvent_t done = nullptr;
+    hipStream_t stream = nullptr, side = nullptr;
+    hipEvent_t done = nullptr, fork = nullptr, join = nullptr;
     BatchLevel* d_levels = nullptr;
     BatchSeg* d_segs = nullptr;
     BatchItem* d_items = nullptr;
-    int* d_scalars = nullptr;  // [0] items, [1] queue head
+    int* d_scalars = nullptr;  // [0] items, [1] first 4x4 item, [2..3] queue heads
     BatchLevel* h_levels = nullptr;  // pinned staging
     BatchSeg* h_segs = nullptr;
     BatchItem* h_items = nullptr;
@@ -150,14 +150,20 @@ int ensure_capacity(astc_server* sv, Slot& s, size_t nlev, size_t nseg, size_t n
     a.levels = s.d_levels;
     a.segs = s.d_segs;
     a.items = s.d_items;
-    a.nitems = s.d_scalars;
-    a.counter = s.d_scalars + 1;
+    a.scalars = s.d_scalars;
+    a.counters = s.d_scalars + 2;
     a.tables = sv->ctx->d_tables;
     a.prefetch = sv->cfg.tuning.prefetch;
-    const int grid = std::max(1, sv->num_cus * sv->cfg.tuning.workgroups_per_cu);
+    BatchLaunch l;
+    l.grid = std::max(1, sv->num_cus * sv->cfg.tuning.workgroups_per_cu);
+    l.grid4 = 2 * l.grid;
+    l.split = sv->cfg.tuning.split_modes;
+    l.side = s.side;
+    l.fork = s.fork;
+    l.join = s.join;
     if ((e = hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
         return hip_fail(e, "hipStreamBeginCapture");
-    hipError_t le = launch_encode_batch(a, grid, s.stream);
+    hipError_t le = launch_encode_batch(a, l, s.stream);
     e = hipStreamEndCapture(s.stream, &s.graph);
     if (le != hipSuccess) return hip_fail(le, "capture of the batch kernel");
     if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
@@ -185,10 +191,11 @@ int geometry(const astc_request& r, ReqGeom* g) {
         L.f.fmt = r.images[i].src_format;
         L.f.flags = r.images[i].flags;
         L.f.aligned = (r.images[i].row_pitch & 3) == 0;
+        L.f.or_mask = normal_or_mask(r.images[i].flags);
         L.dst = reinterpret_cast<uint32_t*>(oo);
         L.xblocks = ig.xblocks;
         L.nblocks = (int)ig.nblocks;
-        g->modes[i] = mode_of(r.images[i].block_dim, r.images[i].flags);
+        g->modes[i] = mode_of(r.images[i].block_dim, r.images[i].src_format, r.images[i].flags, r.images[i].width, r.images[i].height);
         so += align_up((int64_t)r.images[i].row_pitch * r.images[i].height);
         oo += align_up(ig.nblocks * 16);
         g->texels += (int64_t)r.images[i].width * r.images[i].height;
@@ -222,7 +229,8 @@ int astc_server_create(astc_ctx* ctx, const astc_server_config* cfg, astc_server
     if (cfg->hbm_budget_bytes < kAlign || cfg->max_inflight < 1 || cfg->max_inflight > 8 ||
         cfg->max_batch_requests < 1 ||
         !(t.blocks_per_lane == 1 || t.blocks_per_lane == 2 || t.blocks_per_lane == 4 || t.blocks_per_lane == 8) ||
-        t.workgroups_per_cu < 1 || t.workgroups_per_cu > 4 || (t.prefetch != 0 && t.prefetch != 1))
+        t.workgroups_per_cu < 1 || t.workgroups_per_cu > 4 || (t.prefetch != 0 && t.prefetch != 1) ||
+        (t.split_modes != 0 && t.split_modes != 1))
         return fail(ASTC_ERR_INVALID_ARG, "server configuration out of range");
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
@@ -239,9 +247,12 @@ int astc_server_create(astc_ctx* ctx, const astc_server_config* cfg, astc_server
         sv->slots.resize(cfg->max_inflight);
         for (Slot& s : sv->slots) {
             if ((e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking)) != hipSuccess ||
+                (e = hipStreamCreateWithFlags(&s.side, hipStreamNonBlocking)) != hipSuccess ||
                 (e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming)) != hipSuccess ||
-                (e = hipMalloc(&s.d_scalars, 2 * sizeof(int))) != hipSuccess ||
-                (e = hipHostMalloc(&s.h_scalars, 2 * sizeof(int), 0)) != hipSuccess) {
+                (e = hipEventCreateWithFlags(&s.fork, hipEventDisableTiming)) != hipSuccess ||
+                (e = hipEventCreateWithFlags(&s.join, hipEventDisableTiming)) != hipSuccess ||
+                (e = hipMalloc(&s.d_scalars, 4 * sizeof(int))) != hipSuccess ||
+                (e = hipHostMalloc(&s.h_scalars, 4 * sizeof(int), 0)) != hipSuccess) {
                 rc = hip_fail(e, "slot setup");
                 break;
             }
@@ -264,7 +275,11 @@ int astc_server_destroy(astc_server* sv) {
         free_slot_buffers(s);
         if (s.d_scalars) (void)hipFree(s.d_scalars);
         if (s.h_scalars) (void)hipHostFree(s.h_scalars);
+        if (s.side) (void)hipStreamSynchronize(s.side);
         if (s.done) (void)hipEventDestroy(s.done);
+        if (s.fork) (void)hipEventDestroy(s.fork);
+        if (s.join) (void)hipEventDestroy(s.join);
+        if (s.side) (void)hipStreamDestroy(s.side);
         if (s.stream) (void)hipStreamDestroy(s.stream);
     }
     if (sv->ring_base) (void)hipFree(sv->ring_base);
@@ -355,7 +370,8 @@ int astc_server_run_trace(astc_server* sv, const astc_request* reqs, int n, doub
                 }
             }
             if (s.reqs.empty()) break;  // the ring is full: wait for completions
-            plan_work(levels, modes, sv->cfg.tuning.blocks_per_lane, &segs, &items);
+            int first4 = 0;
+            plan_work(levels, modes, sv->cfg.tuning.blocks_per_lane, &segs, &items, &first4);
             if (levels.size() > s.cap_levels || segs.size() > s.cap_segs || items.size() > s.cap_items) {
                 // grow every idle slot at once: regrowing (allocation + graph capture)
                 // costs milliseconds, so it should not recur slot by slot mid-stream
@@ -369,7 +385,9 @@ int astc_server_run_trace(astc_server* sv, const astc_request* reqs, int n, doub
             std::memcpy(s.h_segs, segs.data(), segs.size() * sizeof(BatchSeg));
             std::memcpy(s.h_items, items.data(), items.size() * sizeof(BatchItem));
             s.h_scalars[0] = (int)items.size();
-            s.h_scalars[1] = 0;
+            s.h_scalars[1] = first4;
+            s.h_scalars[2] = 0;
+            s.h_scalars[3] = 0;
             e = hipSuccess;
             for (size_t k = 0; k < s.reqs.size() && e == hipSuccess; ++k) {
                 const int r = s.reqs[k];
@@ -386,7 +404,7 @@ int astc_server_run_trace(astc_server* sv, const astc_request* reqs, int n, doub
                 e = hipMemcpyAsync(s.d_items, s.h_items, items.size() * sizeof(BatchItem), hipMemcpyHostToDevice,
                                    s.stream);
             if (e == hipSuccess)
-                e = hipMemcpyAsync(s.d_scalars, s.h_scalars, 2 * sizeof(int), hipMemcpyHostToDevice, s.stream);
+                e = hipMemcpyAsync(s.d_scalars, s.h_scalars, 4 * sizeof(int), hipMemcpyHostToDevice, s.stream);
             if (e == hipSuccess) e = hipGraphLaunch(s.exec, s.stream);
             for (size_t k = 0; k < s.reqs.size() && e == hipSuccess; ++k) {
                 const int r = s.reqs[k];

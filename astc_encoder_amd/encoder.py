@@ -200,8 +200,8 @@ class Batch:
         check(lib().astc_batch_autotune(self._h, _stream_handle(stream), int(reps), ctypes.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, blocks_per_lane, workgroups_per_cu, prefetch):
-        t = Tuning(blocks_per_lane, workgroups_per_cu, prefetch, 0)
+    def set_tuning(self, blocks_per_lane, workgroups_per_cu, prefetch, split_modes=0):
+        t = Tuning(blocks_per_lane, workgroups_per_cu, prefetch, split_modes)
         check(lib().astc_batch_set_tuning(self._h, ctypes.byref(t)))
 
     def tuning(self):
@@ -267,9 +267,10 @@ class Server:
     """astc_server: continuous batching of requests under an HBM budget."""
 
     def __init__(self, encoder, hbm_budget_bytes, max_inflight=3, max_batch_requests=64,
-                 tuning=(4, 2, 1)):
+                 tuning=(4, 2, 1, 0)):
+        t = tuple(tuning) + (0,) * (4 - len(tuning))
         cfg = ServerConfig(int(hbm_budget_bytes), int(max_inflight), int(max_batch_requests),
-                           Tuning(tuning[0], tuning[1], tuning[2], 0))
+                           Tuning(t[0], t[1], t[2], t[3]))
         self._h = ctypes.c_void_p()
         check(lib().astc_server_create(encoder.handle, ctypes.byref(cfg), ctypes.byref(self._h)))
 

@@ -390,9 +390,8 @@ struct Arith {
 template <typename T> T clamp255(T x) { return std::fmin(std::fmax(x, T(0)), T(255)); }
 
 /* length(v) < SMALL_VALUE, with length = sqrt(dot(v, v)). */
-template <typename T> bool short_vec(const Arith<T>& ar, const T* v) {
-    return std::sqrt(ar.dot(v, v)) < K<T>::small;
-}
+template <typename T> bool short_dot(T dot) { return std::sqrt(dot) < K<T>::small; }
+template <typename T> bool short_vec(const Arith<T>& ar, const T* v) { return short_dot(ar.dot(v, v)); }
 /* Reciprocal square root of the canonical binary32 schedule: seed from the bit
  * pattern, 0x5f375a86 - (bits(x) >> 1), then three Newton steps
  * y = y * fma(-(0.5x * y), y, 1.5).  IEEE operations only, so every machine gets
@@ -425,6 +424,69 @@ template <typename T> void matvec(const Arith<T>& ar, const T m[4][4], const T* 
     for (int i = 0; i < 4; ++i) out[i] = ar.dot(m[i], v);
 }
 
+/* eigen_vector in the canonical schedule.  One iteration of :97-104 is
+ * v <- normalize(M (M v)) after the test length(M v) < eps (which returns M v).
+ * With S = M M (S_ij = dot(row i, row j)), |M v|^2 = dot(v, S v) is the Rayleigh
+ * quotient of the positive semi-definite S at the unit v, which never decreases
+ * along power iterates, so only the test on the starting vector can succeed.
+ * Iterates are rescaled by powers of two (exact; only the direction feeds the
+ * next product) and the eighth is normalised. */
+template <typename T> int exponent_of(T x) {
+    int e;
+    std::frexp(x, &e);
+    return e;
+}
+template <typename T> void eigen_canonical(const Arith<T>& ar, const T m[4][4], T v[4]) {
+    T s2[4][4];
+    for (int i = 0; i < 4; ++i)
+        for (int j = i; j < 4; ++j) s2[i][j] = s2[j][i] = ar.dot(m[i], m[j]);
+    T w[4];
+    matvec(ar, s2, v, w);
+    if (short_dot(ar.dot(v, w))) {
+        T u[4];
+        matvec(ar, m, v, u);
+        std::memcpy(v, u, sizeof(u));
+        return;
+    }
+    for (int it = 1; it < 8; ++it) {
+        const T mx = std::fmax(std::fmax(std::fabs(w[0]), std::fabs(w[1])), std::fmax(std::fabs(w[2]), std::fabs(w[3])));
+        const int e = exponent_of(mx);
+        for (int i = 0; i < 4; ++i) v[i] = std::ldexp(w[i], -e);
+        matvec(ar, s2, v, w);
+    }
+    std::memcpy(v, w, sizeof(w));
+    normalize4(ar, v);
+}
+
+/* calculate_normal_weights (:316-372) in the canonical schedule.  The weight of
+ * grid point i is dot(k, sample_i - e0), rescaled to [0, 1] by the minimum and
+ * range over the grid, so dot(k, e0) cancels and the projections
+ * p_i = dot(k, sample_i) are rescaled instead.  A 6x6 sample is a 4-tap
+ * combination of texels (:307-314), and its projection the same combination of
+ * texel projections. */
+template <typename T>
+void weights_canonical(const Arith<T>& ar, const T* tex, int dim, const T axis[4], T projw[16]) {
+    if (dim == 6) {
+        T pt[36];
+        for (int j = 0; j < 36; ++j) pt[j] = ar.dot(axis, &tex[j * 4]);
+        const Taps& tp = taps6();
+        for (int i = 0; i < 16; ++i) {
+            T s = pt[tp.idx[i][0]] * K<T>::tap[tp.far_count[i][0]];
+            for (int k = 1; k < 4; ++k) s = std::fma(pt[tp.idx[i][k]], K<T>::tap[tp.far_count[i][k]], s);
+            projw[i] = s;
+        }
+    } else {
+        for (int i = 0; i < 16; ++i) projw[i] = ar.dot(axis, &tex[i * 4]);
+    }
+    T minw = K<T>::big, maxw = -K<T>::big;
+    for (int i = 0; i < 16; ++i) {
+        minw = std::fmin(projw[i], minw);
+        maxw = std::fmax(projw[i], maxw);
+    }
+    const T invlen = T(1) / std::fmax(K<T>::small, maxw - minw);
+    for (int i = 0; i < 16; ++i) projw[i] = (projw[i] - minw) * invlen;
+}
+
 template <typename T>
 void encode_block_impl(const T* tex /* N*4 */, int dim, bool has_alpha, int sched, uint8_t out[16],
                        oracle_block_trace* trace) {
@@ -446,22 +508,39 @@ void encode_block_impl(const T* tex /* N*4 */, int dim, bool has_alpha, int sche
 
     /* eigen_vector (:93-106): v = M v; return v if short; v = normalize(M v). */
     T v[4] = {K<T>::v0[0], K<T>::v0[1], K<T>::v0[2], K<T>::v0[3]};
-    for (int it = 0; it < 8; ++it) {
-        T u[4];
-        matvec(ar, cov, v, u);
-        std::memcpy(v, u, sizeof(v));
-        if (short_vec(ar, v)) break;
-        matvec(ar, cov, v, u);
-        normalize4(ar, u);
-        std::memcpy(v, u, sizeof(v));
+    if (ar.canon) {
+        eigen_canonical(ar, cov, v);
+    } else {
+        for (int it = 0; it < 8; ++it) {
+            T u[4];
+            matvec(ar, cov, v, u);
+            std::memcpy(v, u, sizeof(v));
+            if (short_vec(ar, v)) break;
+            matvec(ar, cov, v, u);
+            normalize4(ar, u);
+            std::memcpy(v, u, sizeof(v));
+        }
     }
 
-    /* find_min_max (:108-137) */
+    /* find_min_max (:108-137): t_k = dot(texel_k - mean, v).  Canonical: the
+     * extreme values of dot(texel_k, v), minus dot(mean, v) (rounding is
+     * monotonic, so this equals min/max over k of dot(texel_k, v) - dot(mean, v)). */
     T a = K<T>::big, b = -K<T>::big;
-    for (int k = 0; k < N; ++k) {
-        T t = ar.dot(&d[k * 4], v);
-        a = std::fmin(a, t);
-        b = std::fmax(b, t);
+    if (ar.canon) {
+        for (int k = 0; k < N; ++k) {
+            T t = ar.dot(&tex[k * 4], v);
+            a = std::fmin(a, t);
+            b = std::fmax(b, t);
+        }
+        const T mv = ar.dot(mean, v);
+        a = a - mv;
+        b = b - mv;
+    } else {
+        for (int k = 0; k < N; ++k) {
+            T t = ar.dot(&d[k * 4], v);
+            a = std::fmin(a, t);
+            b = std::fmax(b, t);
+        }
     }
     T e0[4], e1[4];
     for (int c = 0; c < 4; ++c) {
@@ -488,6 +567,9 @@ void encode_block_impl(const T* tex /* N*4 */, int dim, bool has_alpha, int sche
     uint32_t w_q[16];
     if (short_vec(ar, axis)) {
         for (int i = 0; i < 16; ++i) projw[i] = T(0);
+    } else if (ar.canon) {
+        normalize4(ar, axis);
+        weights_canonical(ar, tex, dim, axis, projw);
     } else {
         normalize4(ar, axis);
         T minw = K<T>::big, maxw = -K<T>::big;

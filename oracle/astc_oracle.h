@@ -20,7 +20,10 @@
  *   ORACLE_SCHED_CANONICAL  the documented schedule the HIP kernel implements:
  *                           row-wise fused multiply-add chains combined by a
  *                           fixed tree, reciprocal multiplication for 1/N, a
- *                           Newton reciprocal square root for normalize.
+ *                           Newton reciprocal square root for normalize, and
+ *                           three exact rearrangements (power iteration on M M
+ *                           with one length test, min/max as dot(texel, v) -
+ *                           dot(mean, v), weights from texel projections).
  *   ORACLE_SCHED_SOURCE     the HLSL source order taken literally: sequential
  *                           sums, separate multiply and add, IEEE division.
  * Both run in binary32 (float) or binary64 (double, the FP64 twin).

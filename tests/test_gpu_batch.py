@@ -77,13 +77,14 @@ def test_batch_matches_plain_and_oracle(enc):
     b.close()
 
 
-@pytest.mark.parametrize("bpl,wpc,pf", [(1, 1, 0), (2, 2, 1), (4, 2, 0), (8, 1, 1), (8, 4, 1)])
-def test_every_tuning_gives_the_same_blocks(enc, bpl, wpc, pf):
+@pytest.mark.parametrize("bpl,wpc,pf,split", [(1, 1, 0, 0), (2, 2, 1, 0), (4, 2, 0, 0), (8, 1, 1, 0), (8, 4, 1, 0),
+                                             (1, 1, 0, 1), (2, 2, 1, 1), (4, 2, 1, 1), (8, 4, 0, 1)])
+def test_every_tuning_gives_the_same_blocks(enc, bpl, wpc, pf, split):
     images = mixed_images(enc, 2, 24)
     want = plain_outputs(enc, images)
     b = astc.Batch(enc, images)
-    b.set_tuning(bpl, wpc, pf)
-    assert b.tuning() == {"blocks_per_lane": bpl, "workgroups_per_cu": wpc, "prefetch": pf}
+    b.set_tuning(bpl, wpc, pf, split)
+    assert b.tuning() == {"blocks_per_lane": bpl, "workgroups_per_cu": wpc, "prefetch": pf, "split_modes": split}
     b.launch()
     torch.cuda.synchronize()
     for o, w in zip(b.outputs, want):
@@ -91,11 +92,31 @@ def test_every_tuning_gives_the_same_blocks(enc, bpl, wpc, pf):
     b.close()
 
 
-def test_graph_replays_with_eager_work_between(enc):
+@pytest.mark.parametrize("split", [0, 1])
+def test_graph_replays_with_eager_work_between(enc, split):
     images = mixed_images(enc, 3, 16)
     want = plain_outputs(enc, images)
     b = astc.Batch(enc, images)
+    b.set_tuning(4, 2, 1, split)
     for rep in range(4):
+        for o in b.outputs:
+            o.zero_()
+        torch.cuda.synchronize()
+        b.launch()
+        torch.cuda.synchronize()
+        for o, w in zip(b.outputs, want):
+            np.testing.assert_array_equal(o.cpu().numpy(), w, err_msg=f"replay {rep}")
+    b.close()
+
+
+@pytest.mark.parametrize("dim", [4, 6])
+def test_split_batch_with_one_block_size(enc, dim):
+    """Split launches where one of the two kernels finds no work at all."""
+    images = [(t, dim, f) for t, _, f in mixed_images(enc, 6, 12)]
+    want = plain_outputs(enc, images)
+    b = astc.Batch(enc, images)
+    b.set_tuning(2, 2, 1, 1)
+    for rep in range(2):
         for o in b.outputs:
             o.zero_()
         torch.cuda.synchronize()
@@ -125,6 +146,7 @@ def test_autotune_picks_a_valid_tuning(enc):
     b = astc.Batch(enc, images)
     t = b.autotune(reps=2)
     assert t["blocks_per_lane"] in (1, 2, 4, 8) and t["workgroups_per_cu"] in (1, 2)
+    assert t["split_modes"] in (0, 1)
     for o in b.outputs:
         o.zero_()
     b.launch()

@@ -65,6 +65,32 @@ def test_every_flag_and_format(enc, dim, channels, flags):
     assert_same(gpu_encode(enc, img, dim, flags), orc.oracle_encode(img, dim, flags), f"{dim} {channels} {flags}")
 
 
+def degenerate_image(rng, h, w, channels):
+    """Random texels with zero, constant, single-channel and two-level tiles, so
+    that the reduced-channel encoders meet zero axes, zero projections and ties."""
+    img = rng.integers(0, 256, size=(h, w, channels), dtype=np.uint8)
+    img[:12, :12] = 0
+    img[:12, 12:24] = 255
+    img[12:24, :12] = 77
+    img[12:24, 12:24, 1:] = 40                       # only the first channel varies
+    img[24:36, :12, 0] = 9                           # first channel constant
+    img[24:36, 12:24] = np.where(rng.random((12, 12, 1)) < 0.5, 0, 255).astype(np.uint8)
+    return img
+
+
+@pytest.mark.parametrize("dim", [4, 6])
+@pytest.mark.parametrize("channels,flags", [(3, 0), (3, 1), (3, 4), (3, 5), (2, 2), (2, 3), (3, 2), (4, 2), (4, 6)])
+@pytest.mark.parametrize("shape", [(36, 48), (38, 45)])
+def test_channel_classes(enc, dim, channels, flags, shape):
+    """RGB8 images tiled exactly by their blocks (alpha is 255 in every block) and
+    normal maps (blue = alpha = 255) take reduced-channel encoders, which must give
+    the four-channel result byte for byte; untiled RGB8 takes the full encoder."""
+    rng = np.random.default_rng(dim * 1000 + channels * 100 + flags + shape[1])
+    img = degenerate_image(rng, *shape, channels)
+    print(f"case dim={dim} ch={channels} flags={flags} shape={shape}", flush=True)
+    assert_same(gpu_encode(enc, img, dim, flags), orc.oracle_encode(img, dim, flags), f"{dim} {channels} {flags} {shape}")
+
+
 @pytest.mark.parametrize("channels,dim", [(4, 4), (3, 6), (2, 6), (3, 4)])
 def test_unaligned_source_and_padded_pitch(enc, channels, dim):
     """Source pointer off by one byte and an odd row pitch take the guarded path."""

@@ -169,3 +169,89 @@ def test_canonical_and_source_schedules_agree_on_most_blocks(dim, alpha):
         b, _ = orc.encode_block(tex, dim, alpha, orc.SCHED_SOURCE)
         same += np.array_equal(a, b)
     assert same >= 0.95 * 500
+
+
+def _blocks(rng, dim, count):
+    """Random, two-colour, constant and alpha-only blocks (texel values 0..255)."""
+    n = dim * dim
+    out = []
+    for i in range(count):
+        kind = i % 4
+        if kind == 0:
+            tex = np.clip(rng.uniform(0, 255, 4) + rng.normal(0, 30, (n, 4)), 0, 255).round()
+        elif kind == 1:
+            a, b = rng.integers(0, 256, 4), rng.integers(0, 256, 4)
+            tex = np.where(rng.random(n)[:, None] < 0.5, a, b).astype(float)
+        elif kind == 2:
+            tex = np.tile(rng.integers(0, 256, 4).astype(float), (n, 1))
+        else:
+            tex = np.tile(rng.integers(0, 256, 4).astype(float), (n, 1))
+            tex[:, 3] = rng.integers(0, 256, n)
+        out.append(tex)
+    return out
+
+
+@pytest.mark.parametrize("dim,alpha", [(4, False), (4, True), (6, True), (6, False)])
+def test_canonical_rearrangements_equal_the_shader_formulas_in_binary64(dim, alpha):
+    """The canonical schedule computes the principal axis from S = M M with one
+    length test, min/max as dot(texel, v) - dot(mean, v), and weights from texel
+    projections.  In binary64 these must give the shader's literal formulas'
+    axis, endpoints and weights (up to rounding): a wrong tap, index or dropped
+    term shows up here as a large difference."""
+    rng = np.random.default_rng(21)
+    worst_axis = worst_ep = 0.0
+    diff_blocks = 0
+    blocks = _blocks(rng, dim, 400)
+    for tex in blocks:
+        bc, tc = orc.encode_block(tex, dim, alpha, orc.SCHED_CANONICAL, f64=True)
+        bs, ts = orc.encode_block(tex, dim, alpha, orc.SCHED_SOURCE, f64=True)
+        ac, as_ = np.array(tc.axis), np.array(ts.axis)
+        worst_axis = max(worst_axis, np.max(np.abs(ac - as_)))
+        worst_ep = max(worst_ep, np.max(np.abs(np.array(tc.e0) - np.array(ts.e0))),
+                       np.max(np.abs(np.array(tc.e1) - np.array(ts.e1))))
+        if not np.array_equal(bc, bs):
+            diff_blocks += 1
+            # only a weight on a rounding tie may differ, by one step
+            assert list(tc.ep_q) == list(ts.ep_q)
+            assert np.max(np.abs(np.array(tc.w_q, int) - np.array(ts.w_q, int))) <= 1
+    assert worst_axis < 1e-9
+    assert worst_ep < 1e-9
+    # two-colour 6x6 blocks put grid weights exactly on .5 steps (tap sums such
+    # as 0.444 / 0.888), where the last bit decides the rounding
+    assert diff_blocks <= 0.05 * len(blocks)
+
+
+def test_power_iteration_length_test_only_fires_on_the_first_step():
+    """|M v|^2 = v.(M M)v is the Rayleigh quotient of the PSD matrix M M at the
+    unit iterate v; along v <- normalize(M M v) it never decreases."""
+    rng = np.random.default_rng(3)
+    v0 = np.array([0.26726, 0.80178, 0.53452, 0.0])
+    for _ in range(2000):
+        x = rng.normal(0, rng.uniform(0.01, 50), (16, 4)) * (rng.random(4) < 0.8)
+        d = x - x.mean(0)
+        m = d.T @ d / 15
+        v = v0 / np.linalg.norm(v0)
+        q = [v @ m @ m @ v]
+        for _ in range(8):
+            w = m @ (m @ v)
+            if np.linalg.norm(w) == 0:
+                break
+            v = w / np.linalg.norm(w)
+            q.append(v @ m @ m @ v)
+        q = np.array(q)
+        assert np.all(np.diff(q) >= -1e-9 * max(1.0, q.max()))
+
+
+def test_weight_clamp_never_acts():
+    """(w - min) * fl(1 / max(1e-5, max - min)) is at most 1 + 2^-22 in binary32,
+    so round(w * range) never exceeds range (the kernel omits the clamp)."""
+    rng = np.random.default_rng(8)
+    f = np.float32
+    for rng_scale in (1e-6, 1e-3, 1.0, 300.0):
+        p = (rng.random((20000, 16)) * rng_scale).astype(f)
+        p[:, 0] = p.min(1)  # duplicate extremes
+        lo, hi = p.min(1, keepdims=True), p.max(1, keepdims=True)
+        inv = (f(1) / np.maximum(f(1e-5), hi - lo)).astype(f)
+        wn = ((p - lo).astype(f) * inv).astype(f)
+        for r in (5, 11):
+            assert np.rint((wn * f(r)).astype(f)).max() <= r
