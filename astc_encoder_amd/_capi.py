@@ -18,7 +18,7 @@ ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "out of memory", -4: "uns
 
 FMT_RGBA8, FMT_RGB8, FMT_RG8 = 0, 1, 2
 BYTES_PER_TEXEL = {FMT_RGBA8: 4, FMT_RGB8: 3, FMT_RG8: 2}
-FLAG_ALPHA, FLAG_NORMAL, FLAG_SRGB = 1, 2, 4
+FLAG_ALPHA, FLAG_NORMAL, FLAG_SRGB, FLAG_MAX_ACCUMULATION = 1, 2, 4, 8
 
 
 class AstcError(RuntimeError):

@@ -174,6 +174,8 @@ int astc_batch_create(astc_ctx* ctx, const astc_image_desc* descs, const void* c
     for (int i = 0; i < count; ++i) {
         ImageGeom g;
         int rc = check_desc(&descs[i], &g);
+        if (rc == ASTC_OK && (descs[i].flags & ASTC_FLAG_MAX_ACCUMULATION))
+            rc = fail(ASTC_ERR_UNSUPPORTED, "the max-accumulation search is available in astc_encode only");
         if (rc != ASTC_OK || !srcs[i] || !dsts[i] || (reinterpret_cast<uintptr_t>(dsts[i]) & 15u)) {
             delete b;
             if (rc == ASTC_OK) rc = fail(ASTC_ERR_INVALID_ARG, "image " + std::to_string(i) + ": bad src/dst pointer");

@@ -4,10 +4,9 @@
 // kernel.  It computes what ASTC_Encode.hlsl:510-550 (encode_block) computes,
 // in a fixed floating-point schedule ("canonical schedule") that is reproduced
 // bit for bit by the CPU oracle:
-//   * sums over a block run left to right within each texel row and combine the
-//     row partials as (r0+r1)+(r2+r3) for 4x4 and ((r0+r1)+r2)+((r3+r4)+r5) for 6x6;
-//   * sums of products use fused multiply-add chains along each row, starting
-//     with a plain product;
+//   * sums over a block run left to right over the texels in row-major order;
+//   * sums of products are one fused multiply-add chain over the texels,
+//     starting with a plain product;
 //   * dot products are fma(a3,b3, fma(a2,b2, fma(a1,b1, a0*b0)));
 //   * "x * s + t" expressions of the shader are single fused multiply-adds;
 //   * divisions by the constants N and N-1 are multiplications by the
@@ -32,7 +31,7 @@
 
 namespace astc {
 
-enum : uint32_t { FLAG_ALPHA = 1u, FLAG_NORMAL = 2u, FLAG_SRGB = 4u };
+enum : uint32_t { FLAG_ALPHA = 1u, FLAG_NORMAL = 2u, FLAG_SRGB = 4u, FLAG_MAX_ACCUMULATION = 8u };
 enum : int { FMT_RGBA8 = 0, FMT_RGB8 = 1, FMT_RG8 = 2 };
 
 // Tables staged into LDS by every workgroup.
@@ -78,20 +77,6 @@ __device__ __forceinline__ float ldexp_(float x, int e) { return __builtin_amdgc
 __device__ __forceinline__ bool is_short(float dot) { return dot < __uint_as_float(kLenThresholdBits); }
 
 __device__ __forceinline__ float clamp255(float x) { return fminf(fmaxf(x, 0.0f), 255.0f); }
-
-// Row-partial combination tree.
-template <int DIM>
-struct RowTree {
-    float acc0, acc1;  // DIM 4: (r0+r1), (r2+r3); DIM 6: ((r0+r1)+r2), ((r3+r4)+r5)
-    __device__ __forceinline__ void add_row(int y, float r) {
-        constexpr int half = DIM / 2;
-        if (y == 0) acc0 = r;
-        else if (y < half) acc0 = acc0 + r;
-        else if (y == half) acc1 = r;
-        else acc1 = acc1 + r;
-    }
-    __device__ __forceinline__ float total() const { return acc0 + acc1; }
-};
 
 // ---------------------------------------------------------------------------
 // Fetch (ASTC_Encode.hlsl:561-580 + D3D11 Load): out-of-bounds texels are zero;
@@ -259,7 +244,12 @@ __device__ __forceinline__ Block128 pack_block(const uint32_t e0q[4], const uint
 // the four-channel computation adds only zero terms for it.  The specialised
 // code leaves those terms out; it produces the same bytes.
 // ---------------------------------------------------------------------------
-template <int DIM, bool ALPHA, int NC = 4>
+// METHOD: METHOD_PCA, the shader's principal component analysis (:139-168), or
+// METHOD_MAX_ACCUMULATION, the max_accumulation_pixel_direction search it leaves
+// commented out (:170-227, :514).
+enum : int { METHOD_PCA = 0, METHOD_MAX_ACCUMULATION = 1 };
+
+template <int DIM, bool ALPHA, int NC = 4, int METHOD = METHOD_PCA>
 __device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], const uint8_t* trit_lds) {
     static_assert(NC >= 2 && NC <= 4, "two to four active channels");
     constexpr int N = DIM * DIM;
@@ -270,89 +260,122 @@ __device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], co
     float mean[4] = {255.0f, 255.0f, 255.0f, 255.0f};
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-        RowTree<DIM> tree;
+        float r = x[0][c];
 #pragma unroll
-        for (int y = 0; y < DIM; ++y) {
-            float r = x[y * DIM][c];
-#pragma unroll
-            for (int i = 1; i < DIM; ++i) r = r + x[y * DIM + i][c];
-            tree.add_row(y, r);
-        }
-        mean[c] = tree.total() * rcp_n;
+        for (int k = 1; k < N; ++k) r = r + x[k][c];
+        mean[c] = r * rcp_n;
     }
 
-    // covariance (:149-162): upper triangle of the active channels
-    constexpr int NP = NC * (NC + 1) / 2;
-    float cov[NP];
-    {
-        RowTree<DIM> tree[NP];
+    float v[4];
+    if (METHOD == METHOD_MAX_ACCUMULATION) {
+        // max_accumulation_pixel_direction (:180-224): acc[c] sums the
+        // differences from the mean of the texels whose channel c is above the
+        // mean (an fma with a 0/1 factor adds exactly the difference or nothing);
+        // the axis is the longest acc[c] (alpha competes only with ALPHA),
+        // normalised unless shorter than 1e-5.
+        float acc[4][4];
 #pragma unroll
-        for (int y = 0; y < DIM; ++y) {
-            float r[NP];
+        for (int c = 0; c < NC; ++c)
 #pragma unroll
-            for (int i = 0; i < DIM; ++i) {
-                float d[4];
+            for (int j = 0; j < NC; ++j) acc[c][j] = 0.0f;
 #pragma unroll
-                for (int c = 0; c < NC; ++c) d[c] = x[y * DIM + i][c] - mean[c];
-                int p = 0;
+        for (int k = 0; k < N; ++k) {
+            float d[4];
 #pragma unroll
-                for (int a = 0; a < NC; ++a)
+            for (int c = 0; c < NC; ++c) d[c] = x[k][c] - mean[c];
 #pragma unroll
-                    for (int b2 = a; b2 < NC; ++b2, ++p) r[p] = (i == 0) ? d[a] * d[b2] : fma_(d[a], d[b2], r[p]);
+            for (int c = 0; c < NC; ++c) {
+                const float on = d[c] > 0.0f ? 1.0f : 0.0f;
+#pragma unroll
+                for (int j = 0; j < NC; ++j) acc[c][j] = fma_(on, d[j], acc[c][j]);
             }
-#pragma unroll
-            for (int p = 0; p < NP; ++p) tree[p].add_row(y, r[p]);
         }
+        float maxdot = dotn<NC>(acc[0], acc[0]);
 #pragma unroll
-        for (int p = 0; p < NP; ++p) cov[p] = tree[p].total() * rcp_n1;
-    }
-    float m[4][4];
-    {
-        int p = 0;
+        for (int j = 0; j < NC; ++j) v[j] = acc[0][j];
 #pragma unroll
-        for (int a = 0; a < NC; ++a)
+        for (int c = 1; c < NC; ++c) {
+            if (c == 3 && !ALPHA) break;
+            const float dc = dotn<NC>(acc[c], acc[c]);
+            const bool better = dc > maxdot;
+            maxdot = better ? dc : maxdot;
 #pragma unroll
-            for (int b2 = a; b2 < NC; ++b2, ++p) m[a][b2] = m[b2][a] = cov[p];
-    }
-
-    // eigen_vector (:93-106).  One iteration is v <- normalize(M (M v)), after
-    // the test length(M v) < eps, which returns M v.  With S = M M formed once
-    // (S_ij = dot(row i, row j)) an iteration is v <- normalize(S v), and
-    // |M v|^2 = dot(v, S v) is the Rayleigh quotient of S at the unit v.  S is
-    // positive semi-definite, so that quotient never decreases from one power
-    // iterate to the next: only the test on the starting vector can succeed.
-    // Between iterations v is rescaled by a power of two (exact, and the
-    // direction is all the next product needs); the last step normalises.
-    float s2[4][4];
+            for (int j = 0; j < NC; ++j) v[j] = better ? acc[c][j] : v[j];
+        }
+        if (!is_short(maxdot)) {
+            const float sc = rsqrt_nr(maxdot);
 #pragma unroll
-    for (int i = 0; i < NC; ++i)
-#pragma unroll
-        for (int j = i; j < NC; ++j) s2[i][j] = s2[j][i] = dotn<NC>(m[i], m[j]);
-    float v[4] = {0.26726f, 0.80178f, 0.53452f, 0.0f};
-    float w[4];
-#pragma unroll
-    for (int i = 0; i < NC; ++i) w[i] = dotn<NC>(s2[i], v);
-    if (is_short(dotn<NC>(v, w))) {
-        float u[4];
-#pragma unroll
-        for (int i = 0; i < NC; ++i) u[i] = dotn<NC>(m[i], v);
-#pragma unroll
-        for (int i = 0; i < NC; ++i) v[i] = u[i];
+            for (int j = 0; j < NC; ++j) v[j] = v[j] * sc;
+        }
     } else {
+        // covariance (:149-162): upper triangle of the active channels, one fused
+        // multiply-add chain per entry over the texels
+        constexpr int NP = NC * (NC + 1) / 2;
+        float cov[NP];
 #pragma unroll
-        for (int it = 1; it < 8; ++it) {
-            float mx = fabsf(w[0]);
+        for (int k = 0; k < N; ++k) {
+            float d[4];
 #pragma unroll
-            for (int i = 1; i < NC; ++i) mx = fmaxf(mx, fabsf(w[i]));
-            const int e = exponent_of(mx);
+            for (int c = 0; c < NC; ++c) d[c] = x[k][c] - mean[c];
+            int p = 0;
 #pragma unroll
-            for (int i = 0; i < NC; ++i) v[i] = ldexp_(w[i], -e);
+            for (int a = 0; a < NC; ++a)
 #pragma unroll
-            for (int i = 0; i < NC; ++i) w[i] = dotn<NC>(s2[i], v);
+                for (int b2 = a; b2 < NC; ++b2, ++p) cov[p] = (k == 0) ? d[a] * d[b2] : fma_(d[a], d[b2], cov[p]);
         }
-        const float sc = rsqrt_nr(dotn<NC>(w, w));
 #pragma unroll
-        for (int i = 0; i < NC; ++i) v[i] = w[i] * sc;
+        for (int p = 0; p < NP; ++p) cov[p] = cov[p] * rcp_n1;
+        float m[4][4];
+        {
+            int p = 0;
+#pragma unroll
+            for (int a = 0; a < NC; ++a)
+#pragma unroll
+                for (int b2 = a; b2 < NC; ++b2, ++p) m[a][b2] = m[b2][a] = cov[p];
+        }
+
+        // eigen_vector (:93-106).  One iteration is v <- normalize(M (M v)), after
+        // the test length(M v) < eps, which returns M v.  With S = M M formed once
+        // (S_ij = dot(row i, row j)) an iteration is v <- normalize(S v), and
+        // |M v|^2 = dot(v, S v) is the Rayleigh quotient of S at the unit v.  S is
+        // positive semi-definite, so that quotient never decreases from one power
+        // iterate to the next: only the test on the starting vector can succeed.
+        // Between iterations v is rescaled by a power of two (exact, and the
+        // direction is all the next product needs); the last step normalises.
+        float s2[4][4];
+#pragma unroll
+        for (int i = 0; i < NC; ++i)
+#pragma unroll
+            for (int j = i; j < NC; ++j) s2[i][j] = s2[j][i] = dotn<NC>(m[i], m[j]);
+        v[0] = 0.26726f;
+        v[1] = 0.80178f;
+        v[2] = 0.53452f;
+        v[3] = 0.0f;
+        float w[4];
+#pragma unroll
+        for (int i = 0; i < NC; ++i) w[i] = dotn<NC>(s2[i], v);
+        if (is_short(dotn<NC>(v, w))) {
+            float u[4];
+#pragma unroll
+            for (int i = 0; i < NC; ++i) u[i] = dotn<NC>(m[i], v);
+#pragma unroll
+            for (int i = 0; i < NC; ++i) v[i] = u[i];
+        } else {
+#pragma unroll
+            for (int it = 1; it < 8; ++it) {
+                float mx = fabsf(w[0]);
+#pragma unroll
+                for (int i = 1; i < NC; ++i) mx = fmaxf(mx, fabsf(w[i]));
+                const int e = exponent_of(mx);
+#pragma unroll
+                for (int i = 0; i < NC; ++i) v[i] = ldexp_(w[i], -e);
+#pragma unroll
+                for (int i = 0; i < NC; ++i) w[i] = dotn<NC>(s2[i], v);
+            }
+            const float sc = rsqrt_nr(dotn<NC>(w, w));
+#pragma unroll
+            for (int i = 0; i < NC; ++i) v[i] = w[i] * sc;
+        }
     }
 #pragma unroll
     for (int i = NC; i < 4; ++i) v[i] = 0.0f;

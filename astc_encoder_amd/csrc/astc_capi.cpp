@@ -41,7 +41,7 @@ int check_desc(const astc_image_desc* d, ImageGeom* g) {
     }
     if ((int64_t)d->row_pitch < (int64_t)d->width * bpp)
         return fail(ASTC_ERR_INVALID_ARG, "row_pitch is smaller than width * bytes per texel");
-    if (d->flags & ~(ASTC_FLAG_ALPHA | ASTC_FLAG_NORMAL | ASTC_FLAG_SRGB))
+    if (d->flags & ~(ASTC_FLAG_ALPHA | ASTC_FLAG_NORMAL | ASTC_FLAG_SRGB | ASTC_FLAG_MAX_ACCUMULATION))
         return fail(ASTC_ERR_INVALID_ARG, "unknown flag bits");
     const int xb = (d->width + d->block_dim - 1) / d->block_dim;
     const int yb = (d->height + d->block_dim - 1) / d->block_dim;

@@ -55,13 +55,13 @@ __host__ __device__ inline int channel_class(int fmt, uint32_t flags, int width,
 
 // SRGB: 1 / 0 fixes the fetch at compile time, -1 selects the table per image
 // (sRGB or identity, no branch).  NC: the channel class's active channels.
-template <int DIM, bool ALPHA, int SRGB, int NC>
+template <int DIM, bool ALPHA, int SRGB, int NC, int METHOD = METHOD_PCA>
 __device__ __forceinline__ Block128 encode_texels(const uint32_t p[DIM * DIM], const FetchArgs& f, const float* lut,
                                                   const uint8_t* trit) {
     float x[DIM * DIM][4];
     if (SRGB < 0) unpack_block<DIM, true, NC>(p, f.or_mask, lut + ((f.flags & FLAG_SRGB) ? 0 : 256), x);
     else unpack_block<DIM, SRGB == 1, NC>(p, f.or_mask, lut, x);
-    return encode_block<DIM, ALPHA, NC>(x, trit);
+    return encode_block<DIM, ALPHA, NC, METHOD>(x, trit);
 }
 
 __device__ __forceinline__ void store_block(uint32_t* dst, int id, const Block128& b) {
@@ -72,7 +72,7 @@ __device__ __forceinline__ void store_block(uint32_t* dst, int id, const Block12
 // threads: step k, thread t takes block first + k * blockDim.x + t.  With
 // prefetch the next step's texels are loaded before the current block is
 // encoded.
-template <int DIM, bool ALPHA, int SRGB, int CH>
+template <int DIM, bool ALPHA, int SRGB, int CH, int METHOD = METHOD_PCA>
 __device__ __forceinline__ void encode_chunk(const FetchArgs& f, int xblocks, uint32_t* dst, int first, int count,
                                              bool prefetch, const float* lut, const uint8_t* trit) {
     const int stride = blockDim.x;
@@ -85,7 +85,7 @@ __device__ __forceinline__ void encode_chunk(const FetchArgs& f, int xblocks, ui
         const bool has_next = o + stride < count;
         uint32_t nxt[DIM * DIM];
         if (has_next && prefetch) load_block_id<DIM>(f, xblocks, first + o + stride, nxt);
-        store_block(dst, first + o, encode_texels<DIM, ALPHA, SRGB, CH>(cur, f, lut, trit));
+        store_block(dst, first + o, encode_texels<DIM, ALPHA, SRGB, CH, METHOD>(cur, f, lut, trit));
         if (!has_next) break;
         if (prefetch) {
 #pragma unroll
@@ -101,7 +101,7 @@ __device__ __forceinline__ void encode_chunk(const FetchArgs& f, int xblocks, ui
 template <int DIM>
 struct PlainWaves { static constexpr int value = DIM == 4 ? 4 : 2; };
 
-template <int DIM, bool ALPHA, bool SRGB, int CH>
+template <int DIM, bool ALPHA, bool SRGB, int CH, int METHOD>
 __global__ void __launch_bounds__(kPlainThreads)
 __attribute__((amdgpu_waves_per_eu(PlainWaves<DIM>::value))) encode_plain_kernel(PlainArgs a, int bpl) {
     __shared__ float lut[512];
@@ -111,7 +111,7 @@ __attribute__((amdgpu_waves_per_eu(PlainWaves<DIM>::value))) encode_plain_kernel
     const int chunk = kPlainThreads * bpl;
     const int first = blockIdx.x * chunk;
     const int count = min(chunk, a.nblocks - first);
-    encode_chunk<DIM, ALPHA, SRGB ? 1 : 0, CH>(a.f, a.xblocks, a.dst, first, count, true, lut, trit);
+    encode_chunk<DIM, ALPHA, SRGB ? 1 : 0, CH, METHOD>(a.f, a.xblocks, a.dst, first, count, true, lut, trit);
 }
 
 // ---------------------------------------------------------------------------
@@ -255,17 +255,23 @@ hipError_t launch_downsample(const uint8_t* src, int w, int h, int bpp, uint8_t*
     return hipGetLastError();
 }
 
-template <int DIM, bool ALPHA, int CH>
+template <int DIM, bool ALPHA, int CH, int METHOD = METHOD_PCA>
 static void launch_plain_ch(const PlainArgs& a, const dim3 grid, int bpl, hipStream_t stream) {
     if (a.f.flags & FLAG_SRGB)
-        hipLaunchKernelGGL((encode_plain_kernel<DIM, ALPHA, true, CH>), grid, dim3(kPlainThreads), 0, stream, a, bpl);
+        hipLaunchKernelGGL((encode_plain_kernel<DIM, ALPHA, true, CH, METHOD>), grid, dim3(kPlainThreads), 0, stream, a,
+                           bpl);
     else
-        hipLaunchKernelGGL((encode_plain_kernel<DIM, ALPHA, false, CH>), grid, dim3(kPlainThreads), 0, stream, a, bpl);
+        hipLaunchKernelGGL((encode_plain_kernel<DIM, ALPHA, false, CH, METHOD>), grid, dim3(kPlainThreads), 0, stream, a,
+                           bpl);
 }
 
 template <int DIM, bool ALPHA>
 static void launch_plain_t(const PlainArgs& a, int64_t nblocks, int bpl, hipStream_t stream) {
     const dim3 grid(unsigned((nblocks + kPlainThreads * bpl - 1) / (kPlainThreads * bpl)));
+    if (a.f.flags & FLAG_MAX_ACCUMULATION) {  // optional search: four-channel encoder only
+        launch_plain_ch<DIM, ALPHA, CH_ALL, METHOD_MAX_ACCUMULATION>(a, grid, bpl, stream);
+        return;
+    }
     switch (channel_class(a.f.fmt, a.f.flags, a.f.width, a.f.height, DIM)) {
     case CH_NORMAL: launch_plain_ch<DIM, ALPHA, CH_NORMAL>(a, grid, bpl, stream); break;
     case CH_RGB8: launch_plain_ch<DIM, ALPHA, CH_RGB8>(a, grid, bpl, stream); break;

@@ -13,7 +13,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._capi import (FLAG_ALPHA, FLAG_NORMAL, FLAG_SRGB, FMT_RG8, FMT_RGB8, FMT_RGBA8, ImageDesc,
+from ._capi import (FLAG_ALPHA, FLAG_MAX_ACCUMULATION, FLAG_NORMAL, FLAG_SRGB, FMT_RG8, FMT_RGB8, FMT_RGBA8, ImageDesc,
                     BYTES_PER_TEXEL, Request, ServerConfig, ServerStats, Tuning, check, lib)
 
 _CHANNELS_TO_FMT = {4: FMT_RGBA8, 3: FMT_RGB8, 2: FMT_RG8}
@@ -32,6 +32,9 @@ class EncodeOption:
     is_normal_map: bool = False
     has_alpha: bool = False
     srgb: bool = False
+    # endpoint search of ASTC_Encode.hlsl:170-227, commented out in the reference
+    # (:514); not part of its options
+    max_accumulation: bool = False
 
     @property
     def block_dim(self):
@@ -47,6 +50,8 @@ class EncodeOption:
             f |= FLAG_NORMAL
         if self.srgb and not self.is_normal_map:
             f |= FLAG_SRGB
+        if self.max_accumulation:
+            f |= FLAG_MAX_ACCUMULATION
         return f
 
 

@@ -50,6 +50,13 @@ extern "C" {
 #define ASTC_FLAG_ALPHA 1u
 #define ASTC_FLAG_NORMAL 2u
 #define ASTC_FLAG_SRGB 4u
+/* Optional endpoint search: max_accumulation_pixel_direction (ASTC_Encode.hlsl:
+ * 170-227), which the reference leaves commented out in favour of the principal
+ * component analysis (:514).  Per channel, the sum of the texel differences from
+ * the mean over the texels whose channel is above the mean; the longest sum is
+ * the axis.  astc_encode / astc_encode_each only (the batch and serving planners
+ * reject it with ASTC_ERR_UNSUPPORTED). */
+#define ASTC_FLAG_MAX_ACCUMULATION 8u
 
 typedef struct astc_image_desc {
     int32_t width;      /* texels, 1 .. 2^24-1 (the .astc header's 24-bit size) */

@@ -348,38 +348,24 @@ struct Arith {
         if (canon) return std::fma(a[3], b[3], std::fma(a[2], b[2], std::fma(a[1], b[1], a[0] * b[0])));
         return ((a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]) + a[3] * b[3];
     }
-    /* Sum over the block's texels of f(k).
-     * canonical: per texel row a left-to-right chain, rows combined as
-     *   dim 4: (r0 + r1) + (r2 + r3)      dim 6: ((r0 + r1) + r2) + ((r3 + r4) + r5)
-     * source: left-to-right over k from 0 (ASTC_Encode.hlsl:143-146). */
+    /* Sum over the block's texels of f(k), left to right over k (both schedules;
+     * ASTC_Encode.hlsl:143-146). */
     template <typename F> T block_sum(int dim, F f) const {
-        if (!canon) {
-            T s = 0;
-            for (int k = 0; k < dim * dim; ++k) s = s + f(k);
-            return s;
-        }
-        T r[6] = {0, 0, 0, 0, 0, 0};
-        for (int y = 0; y < dim; ++y) {
-            r[y] = f(y * dim);
-            for (int x = 1; x < dim; ++x) r[y] = r[y] + f(y * dim + x);
-        }
-        return dim == 4 ? (r[0] + r[1]) + (r[2] + r[3]) : ((r[0] + r[1]) + r[2]) + ((r[3] + r[4]) + r[5]);
+        T s = f(0);
+        for (int k = 1; k < dim * dim; ++k) s = s + f(k);
+        return s;
     }
-    /* Sum over texels of a[k]*b[k]: canonical rows are fma chains starting with a product;
-     * source adds separately rounded products (ASTC_Encode.hlsl:151-161). */
+    /* Sum over texels of a[k]*b[k]: canonical is one fma chain starting with a
+     * product; source adds separately rounded products (ASTC_Encode.hlsl:151-161). */
     T block_dot(int dim, const T* a, const T* b, int stride) const {
         if (!canon) {
             T s = 0;
             for (int k = 0; k < dim * dim; ++k) s = s + a[k * stride] * b[k * stride];
             return s;
         }
-        T r[6] = {0, 0, 0, 0, 0, 0};
-        for (int y = 0; y < dim; ++y) {
-            int k = y * dim;
-            r[y] = a[k * stride] * b[k * stride];
-            for (int x = 1; x < dim; ++x) r[y] = std::fma(a[(k + x) * stride], b[(k + x) * stride], r[y]);
-        }
-        return dim == 4 ? (r[0] + r[1]) + (r[2] + r[3]) : ((r[0] + r[1]) + r[2]) + ((r[3] + r[4]) + r[5]);
+        T r = a[0] * b[0];
+        for (int k = 1; k < dim * dim; ++k) r = std::fma(a[k * stride], b[k * stride], r);
+        return r;
     }
     /* x / n for the constant divisors of :147 and :162.  FXC turns a division by
      * a literal into a multiplication by its reciprocal; the canonical schedule
@@ -487,9 +473,35 @@ void weights_canonical(const Arith<T>& ar, const T* tex, int dim, const T axis[4
     for (int i = 0; i < 16; ++i) projw[i] = (projw[i] - minw) * invlen;
 }
 
+/* max_accumulation_pixel_direction (:170-227), the endpoint search the shader
+ * leaves commented out (:514): for each channel c, the sum of the texel
+ * differences from the mean over the texels whose channel c is above the mean;
+ * the axis is the longest of these sums (alpha competes only with has_alpha),
+ * normalised unless shorter than 1e-5.  Sums run left to right over the texels
+ * in both schedules. */
+template <typename T>
+void max_accumulation_axis(const Arith<T>& ar, const T* d, int N, bool has_alpha, T v[4]) {
+    T sum[4][4] = {};
+    for (int k = 0; k < N; ++k)
+        for (int c = 0; c < 4; ++c)
+            if (d[k * 4 + c] > T(0))
+                for (int j = 0; j < 4; ++j) sum[c][j] = sum[c][j] + d[k * 4 + j];
+    int best = 0;
+    T maxdot = ar.dot(sum[0], sum[0]);
+    for (int c = 1; c < (has_alpha ? 4 : 3); ++c) {
+        const T dc = ar.dot(sum[c], sum[c]);
+        if (dc > maxdot) {
+            maxdot = dc;
+            best = c;
+        }
+    }
+    std::memcpy(v, sum[best], 4 * sizeof(T));
+    if (!short_vec(ar, v)) normalize4(ar, v);
+}
+
 template <typename T>
 void encode_block_impl(const T* tex /* N*4 */, int dim, bool has_alpha, int sched, uint8_t out[16],
-                       oracle_block_trace* trace) {
+                       oracle_block_trace* trace, int method = ORACLE_METHOD_PCA) {
     const Arith<T> ar{sched == ORACLE_SCHED_CANONICAL};
     const int N = dim * dim;
 
@@ -508,7 +520,9 @@ void encode_block_impl(const T* tex /* N*4 */, int dim, bool has_alpha, int sche
 
     /* eigen_vector (:93-106): v = M v; return v if short; v = normalize(M v). */
     T v[4] = {K<T>::v0[0], K<T>::v0[1], K<T>::v0[2], K<T>::v0[3]};
-    if (ar.canon) {
+    if (method == ORACLE_METHOD_MAX_ACCUMULATION) {
+        max_accumulation_axis(ar, d.data(), N, has_alpha, v);
+    } else if (ar.canon) {
         eigen_canonical(ar, cov, v);
     } else {
         for (int it = 0; it < 8; ++it) {
@@ -904,15 +918,25 @@ extern "C" {
 
 int oracle_encode_block_f32(const float* texels, int dim, int has_alpha, int sched, uint8_t out[16],
                             oracle_block_trace* trace) {
-    if (dim != 4 && dim != 6) return -1;
-    encode_block_impl<float>(texels, dim, has_alpha != 0, sched, out, trace);
-    return 0;
+    return oracle_encode_block_method_f32(texels, dim, has_alpha, sched, ORACLE_METHOD_PCA, out, trace);
 }
 
 int oracle_encode_block_f64(const double* texels, int dim, int has_alpha, int sched, uint8_t out[16],
                             oracle_block_trace* trace) {
+    return oracle_encode_block_method_f64(texels, dim, has_alpha, sched, ORACLE_METHOD_PCA, out, trace);
+}
+
+int oracle_encode_block_method_f32(const float* texels, int dim, int has_alpha, int sched, int method,
+                                   uint8_t out[16], oracle_block_trace* trace) {
     if (dim != 4 && dim != 6) return -1;
-    encode_block_impl<double>(texels, dim, has_alpha != 0, sched, out, trace);
+    encode_block_impl<float>(texels, dim, has_alpha != 0, sched, out, trace, method);
+    return 0;
+}
+
+int oracle_encode_block_method_f64(const double* texels, int dim, int has_alpha, int sched, int method,
+                                   uint8_t out[16], oracle_block_trace* trace) {
+    if (dim != 4 && dim != 6) return -1;
+    encode_block_impl<double>(texels, dim, has_alpha != 0, sched, out, trace, method);
     return 0;
 }
 
@@ -932,6 +956,7 @@ int64_t oracle_encode_image(const uint8_t* src, int width, int height, int row_p
     const int xb = (width + dim - 1) / dim, yb = (height + dim - 1) / dim;   /* astc_encode.h:127-128 */
     const int64_t nblk = (int64_t)xb * yb;
     const bool alpha = flags & ORACLE_FLAG_ALPHA;
+    const int method = (flags & ORACLE_FLAG_MAX_ACCUMULATION) ? ORACLE_METHOD_MAX_ACCUMULATION : ORACLE_METHOD_PCA;
     (void)tables(); (void)taps6(); (void)srgb_lut();
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);
@@ -943,7 +968,7 @@ int64_t oracle_encode_image(const uint8_t* src, int width, int height, int row_p
         float tex[36 * 4];
         int by = (int)(id / xb), bx = (int)(id - (int64_t)by * xb);       /* :568-569 */
         fetch_block<float>(src, width, height, row_pitch, src_format, dim, flags, bx, by, tex);
-        encode_block_impl<float>(tex, dim, alpha, sched, out_blocks + 16 * id, nullptr);
+        encode_block_impl<float>(tex, dim, alpha, sched, out_blocks + 16 * id, nullptr, method);
     }
     return nblk;
 }

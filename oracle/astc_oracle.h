@@ -18,8 +18,8 @@
  * it is compiled by FXC without IEEE strictness, which may fuse multiply-adds
  * and fold divisions by constants):
  *   ORACLE_SCHED_CANONICAL  the documented schedule the HIP kernel implements:
- *                           row-wise fused multiply-add chains combined by a
- *                           fixed tree, reciprocal multiplication for 1/N, a
+ *                           sequential sums, one fused multiply-add chain per
+ *                           covariance entry, reciprocal multiplication for 1/N, a
  *                           Newton reciprocal square root for normalize, and
  *                           three exact rearrangements (power iteration on M M
  *                           with one length test, min/max as dot(texel, v) -
@@ -39,7 +39,10 @@ extern "C" {
 
 enum { ORACLE_SCHED_CANONICAL = 0, ORACLE_SCHED_SOURCE = 1 };
 enum { ORACLE_FMT_RGBA8 = 0, ORACLE_FMT_RGB8 = 1, ORACLE_FMT_RG8 = 2 };
-enum { ORACLE_FLAG_ALPHA = 1, ORACLE_FLAG_NORMAL = 2, ORACLE_FLAG_SRGB = 4 };
+enum { ORACLE_FLAG_ALPHA = 1, ORACLE_FLAG_NORMAL = 2, ORACLE_FLAG_SRGB = 4, ORACLE_FLAG_MAX_ACCUMULATION = 8 };
+/* Endpoint search: the shader's principal component analysis (:139-168), or the
+ * max_accumulation_pixel_direction alternative it leaves commented out (:170-227, :514). */
+enum { ORACLE_METHOD_PCA = 0, ORACLE_METHOD_MAX_ACCUMULATION = 1 };
 
 /* Per-block intermediate values, for known-answer tests and FP64 error statements. */
 typedef struct {
@@ -56,6 +59,11 @@ int oracle_encode_block_f32(const float* texels, int dim, int has_alpha, int sch
                             uint8_t out[16], oracle_block_trace* trace);
 int oracle_encode_block_f64(const double* texels, int dim, int has_alpha, int sched,
                             uint8_t out[16], oracle_block_trace* trace);
+
+int oracle_encode_block_method_f32(const float* texels, int dim, int has_alpha, int sched, int method,
+                                   uint8_t out[16], oracle_block_trace* trace);
+int oracle_encode_block_method_f64(const double* texels, int dim, int has_alpha, int sched, int method,
+                                   uint8_t out[16], oracle_block_trace* trace);
 
 /* Fetch one block's texels exactly as MainCS does (ASTC_Encode.hlsl:561-580):
  * out-of-bounds texels are zero, normal maps force b = a = 1, then x255. */

@@ -11,6 +11,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_PATH = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
 
 SCHED_CANONICAL, SCHED_SOURCE = 0, 1
+METHOD_PCA, METHOD_MAX_ACCUMULATION = 0, 1
+FLAG_MAX_ACCUMULATION = 8
 FMT = {4: 0, 3: 1, 2: 2}
 
 P = ctypes.c_void_p
@@ -40,6 +42,8 @@ def lib():
         sig = {
             "oracle_encode_block_f32": (I, [P, I, I, I, P, P]),
             "oracle_encode_block_f64": (I, [P, I, I, I, P, P]),
+            "oracle_encode_block_method_f32": (I, [P, I, I, I, I, P, P]),
+            "oracle_encode_block_method_f64": (I, [P, I, I, I, I, P, P]),
             "oracle_fetch_block_f32": (None, [P, I, I, I, I, I, I, I, I, P]),
             "oracle_fetch_block_f64": (None, [P, I, I, I, I, I, I, I, I, P]),
             "oracle_encode_image": (ctypes.c_int64, [P, I, I, I, I, I, I, I, I, P]),
@@ -80,14 +84,14 @@ def oracle_encode(img, dim, flags, sched=SCHED_CANONICAL, threads=0):
     return out
 
 
-def encode_block(texels, dim, has_alpha, sched=SCHED_CANONICAL, f64=False):
+def encode_block(texels, dim, has_alpha, sched=SCHED_CANONICAL, f64=False, method=METHOD_PCA):
     """Encode one block from fetched texels (N, 4); returns (block bytes, Trace)."""
     dt = np.float64 if f64 else np.float32
     t = np.ascontiguousarray(texels, dtype=dt).reshape(-1)
     out = np.zeros(16, np.uint8)
     tr = Trace()
-    fn = lib().oracle_encode_block_f64 if f64 else lib().oracle_encode_block_f32
-    assert fn(_ptr(t), dim, int(has_alpha), sched, _ptr(out), ctypes.byref(tr)) == 0
+    fn = lib().oracle_encode_block_method_f64 if f64 else lib().oracle_encode_block_method_f32
+    assert fn(_ptr(t), dim, int(has_alpha), sched, method, _ptr(out), ctypes.byref(tr)) == 0
     return out, tr
 
 

@@ -41,7 +41,7 @@ def test_output_size(w, h, dim, expect):
 
 
 @pytest.mark.parametrize("kw", [dict(width=0), dict(height=-1), dict(width=1 << 24), dict(block_dim=5),
-                                dict(src_format=7), dict(flags=8), dict(row_pitch=3)])
+                                dict(src_format=7), dict(flags=16), dict(row_pitch=3)])
 def test_invalid_descriptions_are_rejected(kw):
     args = dict(width=16, height=16, src_format=astc.FMT_RGBA8, block_dim=4, flags=0, row_pitch=64)
     args.update(kw)

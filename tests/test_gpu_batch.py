@@ -160,3 +160,5 @@ def test_batch_rejects_bad_images(enc):
     t = torch.zeros((8, 8, 4), dtype=torch.uint8, device="cuda")
     with pytest.raises(astc.AstcError):
         astc.Batch(enc, [(t, 5, 0)])
+    with pytest.raises(astc.AstcError, match="max-accumulation"):
+        astc.Batch(enc, [(t, 4, astc.FLAG_MAX_ACCUMULATION)])

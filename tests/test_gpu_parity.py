@@ -74,7 +74,8 @@ def degenerate_image(rng, h, w, channels):
     img[12:24, :12] = 77
     img[12:24, 12:24, 1:] = 40                       # only the first channel varies
     img[24:36, :12, 0] = 9                           # first channel constant
-    img[24:36, 12:24] = np.where(rng.random((12, 12, 1)) < 0.5, 0, 255).astype(np.uint8)
+    two = img[24:36, 12:24]
+    two[...] = np.where(rng.random(two.shape[:2] + (1,)) < 0.5, 0, 255).astype(np.uint8)
     return img
 
 
@@ -89,6 +90,18 @@ def test_channel_classes(enc, dim, channels, flags, shape):
     img = degenerate_image(rng, *shape, channels)
     print(f"case dim={dim} ch={channels} flags={flags} shape={shape}", flush=True)
     assert_same(gpu_encode(enc, img, dim, flags), orc.oracle_encode(img, dim, flags), f"{dim} {channels} {flags} {shape}")
+
+
+@pytest.mark.parametrize("dim", [4, 6])
+@pytest.mark.parametrize("channels,flags", [(4, 1), (4, 5), (3, 0), (3, 4), (2, 2), (4, 0)])
+@pytest.mark.parametrize("shape", [(36, 48), (29, 53)])
+def test_max_accumulation_search(enc, dim, channels, flags, shape):
+    """The optional endpoint search (ASTC_Encode.hlsl:170-227) against the oracle's."""
+    rng = np.random.default_rng(dim * 7 + channels * 3 + flags + shape[0])
+    img = degenerate_image(rng, *shape, channels)
+    f = flags | astc.FLAG_MAX_ACCUMULATION
+    print(f"case dim={dim} ch={channels} flags={f} shape={shape}", flush=True)
+    assert_same(gpu_encode(enc, img, dim, f), orc.oracle_encode(img, dim, f), f"{dim} {channels} {f} {shape}")
 
 
 @pytest.mark.parametrize("channels,dim", [(4, 4), (3, 6), (2, 6), (3, 4)])
