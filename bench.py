@@ -271,7 +271,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "encode_plain_kernel<6,true>", "kernel_ms": round(r["kernel_ms"], 4),
+                     "kernel": "encode_plain_kernel<6, true, true, 4, 0>", "kernel_ms": round(r["kernel_ms"], 4),
                      "algorithmic_bytes": algo_bytes},
         "roofline_valu": {"achieved": round(achieved_tf, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                           "frac": round(achieved_tf / FP32_PEAK_TFLOPS, 4), "flops_per_block":
