@@ -62,7 +62,7 @@ class ServerStats(ctypes.Structure):
     _fields_ = [("requests", ctypes.c_int64), ("images", ctypes.c_int64), ("texels", ctypes.c_int64),
                 ("blocks", ctypes.c_int64), ("batches", ctypes.c_int64), ("wall_s", ctypes.c_double),
                 ("mean_batch_requests", ctypes.c_double), ("peak_ring_bytes", ctypes.c_int64),
-                ("peak_inflight", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("peak_inflight", ctypes.c_int32), ("reserved", ctypes.c_int32), ("max_loop_gap_s", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

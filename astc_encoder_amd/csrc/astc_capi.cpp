@@ -84,6 +84,10 @@ int astc_ctx_create(int device, astc_ctx** out) {
     astc_ctx* ctx = new (std::nothrow) astc_ctx();
     if (!ctx) return fail(ASTC_ERR_NO_MEMORY, "host allocation failed");
     ctx->device = device;
+    if ((e = hipDeviceGetAttribute(&ctx->num_cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess) {
+        delete ctx;
+        return hip_fail(e, "hipDeviceGetAttribute(multiprocessor count)");
+    }
     build_tables(&ctx->h_tables);
     if ((e = hipMalloc(&ctx->d_tables, sizeof(DeviceTables))) != hipSuccess) {
         delete ctx;
@@ -125,7 +129,7 @@ int astc_encode(astc_ctx* ctx, const astc_image_desc* desc, const void* src_dev,
     if (reinterpret_cast<uintptr_t>(dst_dev) & 15u) return fail(ASTC_ERR_INVALID_ARG, "dst must be 16-byte aligned");
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-    e = launch_encode_plain(ctx->d_tables, *desc, g, src_dev, dst_dev, static_cast<hipStream_t>(stream));
+    e = launch_encode_plain(ctx->d_tables, *desc, g, src_dev, dst_dev, static_cast<hipStream_t>(stream), ctx->num_cus);
     if (e != hipSuccess) return hip_fail(e, "encode launch");
     return ASTC_OK;
 }

@@ -15,6 +15,7 @@ struct astc_ctx {
     astc::DeviceTables* d_tables = nullptr;  // device copy of the sRGB / trit tables
     astc::DeviceTables h_tables{};           // host copy
     astc::TritQuintDecodeTable* d_dec_tables = nullptr;  // decoder tables
+    int num_cus = 256;                       // compute units of the device
     std::mutex mu;                           // guards lazily created objects
 };
 
@@ -94,10 +95,9 @@ hipError_t launch_downsample(const uint8_t* src, int w, int h, int bpp, uint8_t*
 hipError_t launch_decode(const TritQuintDecodeTable* d_tables, const void* blocks, void* rgba, int width, int height,
                          int bw, int bh, hipStream_t stream);
 
-// Plain per-texture launch (one lane per block).
-// bpl_override: blocks per lane (1, 2, 4 or 8); 0 chooses from the block count.
+// Plain per-texture launch (one lane per block) on a device with num_cus CUs.
 hipError_t launch_encode_plain(const DeviceTables* d_tables, const astc_image_desc& d, const ImageGeom& g,
-                               const void* src, void* dst, hipStream_t stream, int bpl_override = 0);
-int plain_blocks_per_lane(int64_t nblocks, int dim);
+                               const void* src, void* dst, hipStream_t stream, int num_cus);
+int plain_blocks_per_lane(int64_t nblocks, int dim, int num_cus);
 
 }  // namespace astc

@@ -279,19 +279,19 @@ static void launch_plain_t(const PlainArgs& a, int64_t nblocks, int bpl, hipStre
     }
 }
 
-// Blocks per lane for a plain launch: enough workgroups to give every SIMD of
-// the 256 CUs a few waves, then as many blocks per lane as that allows (up to 8)
-// so that the prefetch of the next block overlaps the current one.
-int plain_blocks_per_lane(int64_t nblocks, int dim) {
-    const int64_t waves_wanted = int64_t(256) * 4 * (dim == 6 ? 2 : 4);
-    const int64_t lanes = waves_wanted * 64;
+// Blocks per lane for a plain launch: the fewest (1, 2, 4 or 8) that let every
+// workgroup be resident at once (6x6: two workgroups of four waves per CU, 4x4:
+// four), so that no second round of workgroup starts (table staging, first
+// fetch) trails the first and each lane's later blocks are prefetched.
+int plain_blocks_per_lane(int64_t nblocks, int dim, int num_cus) {
+    const int64_t slots = int64_t(num_cus) * (dim == 6 ? 2 : 4);
     int bpl = 1;
-    while (bpl < 8 && nblocks >= lanes * bpl * 2) bpl *= 2;
+    while (bpl < 8 && (nblocks + int64_t(kPlainThreads) * bpl - 1) / (int64_t(kPlainThreads) * bpl) > slots) bpl *= 2;
     return bpl;
 }
 
 hipError_t launch_encode_plain(const DeviceTables* d_tables, const astc_image_desc& d, const ImageGeom& g,
-                               const void* src, void* dst, hipStream_t stream, int bpl_override) {
+                               const void* src, void* dst, hipStream_t stream, int num_cus) {
     PlainArgs a;
     a.f.src = static_cast<const uint8_t*>(src);
     a.f.width = d.width;
@@ -306,7 +306,7 @@ hipError_t launch_encode_plain(const DeviceTables* d_tables, const astc_image_de
     a.dst = static_cast<uint32_t*>(dst);
     a.tables = d_tables;
     const bool alpha = d.flags & FLAG_ALPHA;
-    const int bpl = bpl_override > 0 ? bpl_override : plain_blocks_per_lane(g.nblocks, d.block_dim);
+    const int bpl = plain_blocks_per_lane(g.nblocks, d.block_dim, num_cus);
     if (d.block_dim == 4) {
         if (alpha) launch_plain_t<4, true>(a, g.nblocks, bpl, stream);
         else launch_plain_t<4, false>(a, g.nblocks, bpl, stream);

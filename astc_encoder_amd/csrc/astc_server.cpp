@@ -258,7 +258,9 @@ int astc_server_create(astc_ctx* ctx, const astc_server_config* cfg, astc_server
                 rc = hip_fail(e, "slot setup");
                 break;
             }
-            if ((rc = ensure_capacity(sv, s, 64, 256, 256)) != ASTC_OK) break;
+            // work-list room for a batch of ~1 G texels (2 MB per slot), so that
+            // regrowing (reallocation + graph capture, milliseconds) stays rare
+            if ((rc = ensure_capacity(sv, s, 1024, 65536, 65536)) != ASTC_OK) break;
         }
     }
     if (rc != ASTC_OK) {
@@ -319,7 +321,7 @@ int astc_server_run_trace(astc_server* sv, const astc_request* reqs, int n, doub
     using clk = std::chrono::steady_clock;
     const auto t0 = clk::now();
     auto now = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
-    double last_done = 0.0;
+    double last_done = 0.0, last_loop = 0.0;
     int rc = ASTC_OK;
 
     while (done < n && rc == ASTC_OK) {
@@ -427,11 +429,16 @@ int astc_server_run_trace(astc_server* sv, const astc_request* reqs, int n, doub
             progressed = true;
         }
         if (rc != ASTC_OK) break;
-        // idle until the next arrival when nothing is on the GPU
+        // Idle until the next arrival when nothing is on the GPU: a sleep can
+        // overshoot by milliseconds, so only long gaps sleep (waking 2 ms early)
+        // and the rest is polled.
+        const double tl = now();
+        st.max_loop_gap_s = std::max(st.max_loop_gap_s, tl - last_loop);
         if (!progressed && inflight == 0 && waiting.empty() && next < n) {
-            const double gap = reqs[next].arrival_s - now();
-            if (gap > 2e-4) std::this_thread::sleep_for(std::chrono::duration<double>(gap - 1e-4));
+            const double gap = reqs[next].arrival_s - tl;
+            if (gap > 3e-3) std::this_thread::sleep_for(std::chrono::duration<double>(gap - 2e-3));
         }
+        last_loop = now();
     }
     for (Slot& s : sv->slots) (void)hipStreamSynchronize(s.stream);
     if (rc != ASTC_OK) return rc;

@@ -188,7 +188,7 @@ def serving_sweep(enc, rates=(1000, 2000, 5000, 10000, 20000, 50000), seconds=0.
                         "mtexels_s": round(st["texels"] / st["wall_s"] / 1e6, 1),
                         "mean_texels_per_request": int(st["texels"] / n),
                         "batches": st["batches"], "mean_batch_requests": round(st["mean_batch_requests"], 2),
-                        "peak_inflight": st["peak_inflight"]})
+                        "peak_inflight": st["peak_inflight"], "max_loop_gap_ms": round(st["max_loop_gap_s"] * 1e3, 3)})
     srv.close()
     return {"workload": f"Poisson arrivals, {pool_size} mip-chained textures of the mixed distribution (seed 1)",
             "hbm_budget_gb": 64, "max_inflight_batches": 3, "sweep": results}

@@ -196,6 +196,7 @@ typedef struct astc_server_stats {
     int64_t peak_ring_bytes;     /* most ring bytes held at once */
     int32_t peak_inflight;       /* most batches on the GPU at once */
     int32_t reserved;
+    double max_loop_gap_s;       /* longest pass of the serving loop (host stalls show here) */
 } astc_server_stats;
 
 /* Bytes the ring holds for one request (sources + blocks, 256-byte aligned). */
