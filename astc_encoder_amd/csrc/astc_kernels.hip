@@ -54,14 +54,13 @@ __host__ __device__ inline int channel_class(int fmt, uint32_t flags, int width,
 }
 
 // SRGB: 1 / 0 fixes the fetch at compile time, -1 selects the table per image
-// (sRGB or identity, no branch).  NC: the channel class's active channels.
-template <int DIM, bool ALPHA, int SRGB, int NC, int METHOD = METHOD_PCA>
-__device__ __forceinline__ Block128 encode_texels(const uint32_t p[DIM * DIM], const FetchArgs& f, const float* lut,
-                                                  const uint8_t* trit) {
-    float x[DIM * DIM][4];
-    if (SRGB < 0) unpack_block<DIM, true, NC>(p, f.or_mask, lut + ((f.flags & FLAG_SRGB) ? 0 : 256), x);
-    else unpack_block<DIM, SRGB == 1, NC>(p, f.or_mask, lut, x);
-    return encode_block<DIM, ALPHA, NC, METHOD>(x, trit);
+// (sRGB or identity, no branch).  The channel class CH is the number of active
+// channels the encoder works on.
+template <int DIM, int SRGB, int CH>
+__device__ __forceinline__ void unpack_texels(const uint32_t p[DIM * DIM], const FetchArgs& f, const float* lut,
+                                              float x[DIM * DIM][4]) {
+    if (SRGB < 0) unpack_block<DIM, true, CH>(p, f.or_mask, lut + ((f.flags & FLAG_SRGB) ? 0 : 256), x);
+    else unpack_block<DIM, SRGB == 1, CH>(p, f.or_mask, lut, x);
 }
 
 __device__ __forceinline__ void store_block(uint32_t* dst, int id, const Block128& b) {
@@ -70,8 +69,9 @@ __device__ __forceinline__ void store_block(uint32_t* dst, int id, const Block12
 
 // Encode blocks [first, first + count) of one image with the workgroup's
 // threads: step k, thread t takes block first + k * blockDim.x + t.  With
-// prefetch the next step's texels are loaded before the current block is
-// encoded.
+// prefetch the next step's texels are loaded as soon as the current ones are
+// converted (into the same registers), so the loads are in flight while the
+// block is encoded; without, after the block is stored.
 template <int DIM, bool ALPHA, int SRGB, int CH, int METHOD = METHOD_PCA>
 __device__ __forceinline__ void encode_chunk(const FetchArgs& f, int xblocks, uint32_t* dst, int first, int count,
                                              bool prefetch, const float* lut, const uint8_t* trit) {
@@ -83,16 +83,13 @@ __device__ __forceinline__ void encode_chunk(const FetchArgs& f, int xblocks, ui
 #pragma unroll 1
     for (;; o += stride) {
         const bool has_next = o + stride < count;
-        uint32_t nxt[DIM * DIM];
-        if (has_next && prefetch) load_block_id<DIM>(f, xblocks, first + o + stride, nxt);
-        store_block(dst, first + o, encode_texels<DIM, ALPHA, SRGB, CH, METHOD>(cur, f, lut, trit));
+        float x[DIM * DIM][4];
+        unpack_texels<DIM, SRGB, CH>(cur, f, lut, x);
+        // the packed words are dead once converted: the next block's loads reuse them
+        if (has_next && prefetch) load_block_id<DIM>(f, xblocks, first + o + stride, cur);
+        store_block(dst, first + o, encode_block<DIM, ALPHA, CH, METHOD>(x, trit));
         if (!has_next) break;
-        if (prefetch) {
-#pragma unroll
-            for (int i = 0; i < DIM * DIM; ++i) cur[i] = nxt[i];
-        } else {
-            load_block_id<DIM>(f, xblocks, first + o + stride, cur);
-        }
+        if (!prefetch) load_block_id<DIM>(f, xblocks, first + o + stride, cur);
     }
 }
 
@@ -131,7 +128,8 @@ __device__ __forceinline__ void process_item(const BatchArgs& a, const BatchItem
     if (item.nseg == 1) {
         const BatchSeg sg = a.segs[item.seg_begin];
         const BatchLevel& L = a.levels[sg.level];
-        encode_chunk<DIM, ALPHA, -1, CH>(L.f, L.xblocks, L.dst, sg.first_block, sg.count, a.prefetch != 0, lut, trit);
+        encode_chunk<DIM, ALPHA, -1, CH>(L.f, L.xblocks, L.dst, sg.first_block, sg.count, a.prefetch != 0, lut,
+                                         trit);
         return;
     }
     int o = threadIdx.x;
@@ -141,7 +139,9 @@ __device__ __forceinline__ void process_item(const BatchArgs& a, const BatchItem
             const BatchLevel& L = a.levels[sg.level];
             uint32_t p[DIM * DIM];
             load_block_id<DIM>(L.f, L.xblocks, sg.first_block + o, p);
-            store_block(L.dst, sg.first_block + o, encode_texels<DIM, ALPHA, -1, CH>(p, L.f, lut, trit));
+            float x[DIM * DIM][4];
+            unpack_texels<DIM, -1, CH>(p, L.f, lut, x);
+            store_block(L.dst, sg.first_block + o, encode_block<DIM, ALPHA, CH>(x, trit));
             return;
         }
         o -= sg.count;
