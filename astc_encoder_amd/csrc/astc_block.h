@@ -188,35 +188,44 @@ __device__ __forceinline__ void unpack_block(const uint32_t p[DIM * DIM], uint32
 // Integer stage: endpoint bytes + weight trits -> 128-bit block
 // (ASTC_Encode.hlsl:400-444, 475-508; ASTC_IntegerSequenceEncoding.hlsl:142-176, 207-276)
 // ---------------------------------------------------------------------------
+// Bits of a 5-value trit group that come from the packed trit code T (8 bits),
+// placed for b low bits per value: m0 T[1:0] m1 T[3:2] m2 T[4] m3 T[6:5] m4 T[7].
+__host__ __device__ constexpr uint32_t trit_group_spread(uint32_t T, int b) {
+    return (T & 3u) << b | ((T >> 2) & 3u) << (2 * b + 2) | ((T >> 4) & 1u) << (3 * b + 4) |
+           ((T >> 5) & 3u) << (4 * b + 5) | (T >> 7) << (5 * b + 7);
+}
+
+// Pack tables in LDS: for each 5-trit tuple index, the spread code bits of its
+// group for QUANT_6 (b = 1) and QUANT_12 (b = 2) weights.
+struct PackTables { uint32_t spread[2][256]; };
+
+// wsh[i]: twice weight i's quantised value in the low five bits (the field
+// offset into the split tables).
 template <bool ALPHA>
-__device__ __forceinline__ Block128 pack_block(const uint32_t e0q[4], const uint32_t e1q[4], const uint32_t wq[16],
-                                               const uint8_t* trit_lds) {
+__device__ __forceinline__ Block128 pack_block(const uint32_t e0q[4], const uint32_t e1q[4], const uint32_t wsh[16],
+                                               const PackTables* pt) {
     constexpr WeightSplit split = ALPHA ? kSplitQ6 : kSplitQ12;
     constexpr int b = split.bits;          // bits per weight below the trit
     constexpr int group_bits = 5 * b + 8;  // one 5-value trit group
+    const uint32_t* spread = pt->spread[ALPHA ? 0 : 1];
     // weight stream, LSB first; at most 4 groups x 18 bits (the fourth only
     // carries value 15, so it ends below bit 64)
     uint64_t stream = 0;
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
-        uint32_t t[5], m[5];
+        uint32_t tidx = 0, low = 0;
+        constexpr int mpos[5] = {0, b + 2, 2 * b + 4, 3 * b + 5, 4 * b + 7};
+        constexpr int pow3[5] = {1, 3, 9, 27, 81};
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
             const int idx = 5 * g + i;
             if (idx < 16) {
-                t[i] = (split.trit_bits >> (2 * wq[idx])) & 3u;
-                m[i] = (split.low_bits >> (2 * wq[idx])) & 3u;
-            } else {
-                t[i] = 0u;
-                m[i] = 0u;
+                // v_bfe_u32 reads its offset from the low five bits
+                tidx += __builtin_amdgcn_ubfe(split.trit_bits, wsh[idx], 2) * pow3[i];
+                low |= __builtin_amdgcn_ubfe(split.low_bits, wsh[idx], 2) << mpos[i];
             }
         }
-        const uint32_t T = trit_lds[t[0] + 3 * t[1] + 9 * t[2] + 27 * t[3] + 81 * t[4]];
-        uint64_t grp = uint64_t(m[0]) | uint64_t(T & 3u) << b | uint64_t(m[1]) << (b + 2) |
-                       uint64_t((T >> 2) & 3u) << (2 * b + 2) | uint64_t(m[2]) << (2 * b + 4) |
-                       uint64_t((T >> 4) & 1u) << (3 * b + 4) | uint64_t(m[3]) << (3 * b + 5) |
-                       uint64_t((T >> 5) & 3u) << (4 * b + 5) | uint64_t(m[4]) << (4 * b + 7) |
-                       uint64_t(T >> 7) << (5 * b + 7);
+        const uint64_t grp = uint64_t(spread[tidx] | low);
         stream |= grp << (g * group_bits);
     }
     // endpoints: QUANT_256 raw bytes r0 r1 g0 g1 b0 b1 [a0 a1] from block bit 17
@@ -250,7 +259,7 @@ __device__ __forceinline__ Block128 pack_block(const uint32_t e0q[4], const uint
 enum : int { METHOD_PCA = 0, METHOD_MAX_ACCUMULATION = 1 };
 
 template <int DIM, bool ALPHA, int NC = 4, int METHOD = METHOD_PCA>
-__device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], const uint8_t* trit_lds) {
+__device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], const PackTables* pack) {
     static_assert(NC >= 2 && NC <= 4, "two to four active channels");
     constexpr int N = DIM * DIM;
     constexpr float rcp_n = 1.0f / float(N);
@@ -429,14 +438,14 @@ __device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], co
     // a fixed 4-tap combination of texels, and so is its projection.
     constexpr uint32_t range = ALPHA ? 5u : 11u;
     constexpr int KC = (NC == 4 && !ALPHA) ? 3 : NC;  // RGB mode: endpoint alphas are equal
-    uint32_t wq[16];
+    uint32_t wsh[16];  // 2 * quantised weight in the low five bits
     float k[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int c = 0; c < KC; ++c) k[c] = e1[c] - e0[c];
     const float kk = dotn<KC>(k, k);
     if (is_short(kk)) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) wq[i] = 0u;
+        for (int i = 0; i < 16; ++i) wsh[i] = 0u;
     } else {
         const float s = rsqrt_nr(kk);
 #pragma unroll
@@ -466,14 +475,17 @@ __device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], co
         }
         // (w - min) * invlen is at most 1 + 2^-22, so round(w * range) never
         // exceeds range and the clamp of quantize_weight (:258) never acts.
+        // round-half-even(y) for 0 <= y < 2^22 is the low bits of y + 1.5 * 2^23
+        // (the sum's unit in the last place is 1); doubled, its low five bits
+        // are 2 * round(y).
         const float invlen = 1.0f / fmaxf(0.00001f, maxw - minw);
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             const float wn = (proj[i] - minw) * invlen;
-            wq[i] = uint32_t(rintf(wn * float(range)));
+            wsh[i] = __float_as_uint(wn * float(range) + 12582912.0f) << 1;
         }
     }
-    return pack_block<ALPHA>(e0q, e1q, wq, trit_lds);
+    return pack_block<ALPHA>(e0q, e1q, wsh, pack);
 }
 
 }  // namespace astc

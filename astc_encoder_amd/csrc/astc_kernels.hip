@@ -25,13 +25,16 @@ struct PlainArgs {
 };
 
 // LDS tables: lut[0..255] sRGB -> float(linear) * 255, lut[256..511] the
-// identity (c -> float(c), the exact linear fetch), trit codes.
-__device__ __forceinline__ void stage_tables(const DeviceTables* t, float* lut, uint8_t* trit) {
+// identity (c -> float(c), the exact linear fetch), and the trit groups' code
+// bits for both weight quantisations.
+__device__ __forceinline__ void stage_tables(const DeviceTables* t, float* lut, PackTables* pack) {
     const int tid = threadIdx.x;
     for (int i = tid; i < 256; i += blockDim.x) {
         lut[i] = t->srgb[i];
         lut[256 + i] = float(i);
-        trit[i] = t->trit[i];
+        const uint32_t T = t->trit[i];
+        pack->spread[0][i] = trit_group_spread(T, kSplitQ6.bits);
+        pack->spread[1][i] = trit_group_spread(T, kSplitQ12.bits);
     }
 }
 
@@ -74,7 +77,7 @@ __device__ __forceinline__ void store_block(uint32_t* dst, int id, const Block12
 // block is encoded; without, after the block is stored.
 template <int DIM, bool ALPHA, int SRGB, int CH, int METHOD = METHOD_PCA>
 __device__ __forceinline__ void encode_chunk(const FetchArgs& f, int xblocks, uint32_t* dst, int first, int count,
-                                             bool prefetch, const float* lut, const uint8_t* trit) {
+                                             bool prefetch, const float* lut, const PackTables* trit) {
     const int stride = blockDim.x;
     int o = threadIdx.x;
     if (o >= count) return;
@@ -102,13 +105,13 @@ template <int DIM, bool ALPHA, bool SRGB, int CH, int METHOD>
 __global__ void __launch_bounds__(kPlainThreads)
 __attribute__((amdgpu_waves_per_eu(PlainWaves<DIM>::value))) encode_plain_kernel(PlainArgs a, int bpl) {
     __shared__ float lut[512];
-    __shared__ uint8_t trit[256];
-    stage_tables(a.tables, lut, trit);
+    __shared__ PackTables trit;
+    stage_tables(a.tables, lut, &trit);
     __syncthreads();
     const int chunk = kPlainThreads * bpl;
     const int first = blockIdx.x * chunk;
     const int count = min(chunk, a.nblocks - first);
-    encode_chunk<DIM, ALPHA, SRGB ? 1 : 0, CH, METHOD>(a.f, a.xblocks, a.dst, first, count, true, lut, trit);
+    encode_chunk<DIM, ALPHA, SRGB ? 1 : 0, CH, METHOD>(a.f, a.xblocks, a.dst, first, count, true, lut, &trit);
 }
 
 // ---------------------------------------------------------------------------
@@ -124,7 +127,7 @@ __global__ void reset_counter_kernel(int* counters) {
 
 template <int DIM, bool ALPHA, int CH>
 __device__ __forceinline__ void process_item(const BatchArgs& a, const BatchItem& item, const float* lut,
-                                             const uint8_t* trit) {
+                                             const PackTables* trit) {
     if (item.nseg == 1) {
         const BatchSeg sg = a.segs[item.seg_begin];
         const BatchLevel& L = a.levels[sg.level];
@@ -150,7 +153,7 @@ __device__ __forceinline__ void process_item(const BatchArgs& a, const BatchItem
 
 template <int DIM, bool ALPHA>
 __device__ __forceinline__ void process_item_ch(const BatchArgs& a, const BatchItem& item, const float* lut,
-                                                const uint8_t* trit) {
+                                                const PackTables* trit) {
     switch (item.mode & MODE_CH_MASK) {
     case MODE_CH_NORMAL: process_item<DIM, ALPHA, CH_NORMAL>(a, item, lut, trit); break;
     case MODE_CH_RGB8: process_item<DIM, ALPHA, CH_RGB8>(a, item, lut, trit); break;
@@ -169,9 +172,9 @@ template <int SEL>
 __global__ void __launch_bounds__(kPlainThreads) __attribute__((amdgpu_waves_per_eu(BatchWaves<SEL>::value)))
 encode_batch_kernel(BatchArgs a) {
     __shared__ float lut[512];
-    __shared__ uint8_t trit[256];
+    __shared__ PackTables trit;
     __shared__ int next_item;
-    stage_tables(a.tables, lut, trit);
+    stage_tables(a.tables, lut, &trit);
     const int nitems = a.scalars[0], first4 = a.scalars[1];
     const int begin = SEL == 4 ? first4 : 0;
     const int end = SEL == 6 ? first4 : nitems;
@@ -188,16 +191,16 @@ encode_batch_kernel(BatchArgs a) {
         // the register budget)
         switch (item.mode & 3) {
         case MODE_4x4_RGB:
-            if (SEL != 6) process_item_ch<4, false>(a, item, lut, trit);
+            if (SEL != 6) process_item_ch<4, false>(a, item, lut, &trit);
             break;
         case MODE_4x4_RGBA:
-            if (SEL != 6) process_item_ch<4, true>(a, item, lut, trit);
+            if (SEL != 6) process_item_ch<4, true>(a, item, lut, &trit);
             break;
         case MODE_6x6_RGB:
-            if (SEL != 4) process_item_ch<6, false>(a, item, lut, trit);
+            if (SEL != 4) process_item_ch<6, false>(a, item, lut, &trit);
             break;
         default:
-            if (SEL != 4) process_item_ch<6, true>(a, item, lut, trit);
+            if (SEL != 4) process_item_ch<6, true>(a, item, lut, &trit);
             break;
         }
     }
