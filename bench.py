@@ -102,7 +102,28 @@ def time_plain(enc, wl, steps, warmup):
     texels = wl["w"] * wl["h"]
     nblocks = out.shape[0]
     return dict(img=img, blocks=out.cpu().numpy(), wall_s=wall, kernel_ms=kernel_ms, texels=texels,
-                nblocks=nblocks, mtexels_s=texels * steps / wall / 1e6)
+                nblocks=nblocks, mtexels_s=texels * steps / wall / 1e6, psnr_db=gpu_psnr(enc, wl, src, out))
+
+
+def gpu_psnr(enc, wl, src, blocks):
+    """PSNR of the encoded texture, decoded by the library's GPU decoder, against the
+    encoder's view of the source (sRGB linearised; the mode's channels: RG for
+    normal maps, RGB, or RGBA)."""
+    import torch
+    import astc_encoder_amd as astc
+    dec = enc.decode(blocks, wl["w"], wl["h"], wl["dim"])
+    nch = 2 if wl["flags"] & astc.FLAG_NORMAL else 4 if wl["flags"] & astc.FLAG_ALPHA else 3
+    ref = src[..., :nch].long()
+    if wl["flags"] & astc.FLAG_SRGB:
+        lut = torch.from_numpy(astc.srgb_table()).to(src.device).double()
+        t = lut[ref]
+        if nch == 4:
+            t[..., 3] = ref[..., 3].double()
+    else:
+        t = ref.double()
+    e = t - dec[..., :nch].double()
+    sse = float((e * e).sum())
+    return round(10.0 * np.log10(255.0 * 255.0 * e.numel() / sse), 4) if sse > 0 else float("inf")
 
 
 def batched_vs_plain(enc, steps, warmup, n_textures=512):
@@ -211,9 +232,11 @@ def cpu_baseline(wl, img, gpu_blocks, budget_s=20.0):
         times.append(time.perf_counter() - t)
     med = float(np.median(times)) if times else first
     match = float(np.mean(np.all(ref == gpu_blocks, axis=1)))
+    # the oracle's own blocks through the oracle's spec decoder: the reference PSNR
+    _, dec = orc.decode_image(ref, wl["w"], wl["h"], wl["dim"])
     return dict(value=round(wl["w"] * wl["h"] / med / 1e6, 2), unit="Mtexels/s", cores=cores, kind="port",
                 sample=f"whole texture ({wl['name']}), median of {max(1, len(times))} runs after 1 warm-up",
-                gpu_block_match=match)
+                gpu_block_match=match, oracle_psnr_db=round(orc.psnr(img, wl["flags"], dec), 4))
 
 
 def main():
@@ -277,14 +300,17 @@ def main():
                           "frac": round(achieved_tf / FP32_PEAK_TFLOPS, 4), "flops_per_block":
                           FLOPS_PER_BLOCK[head["flops_key"]]},
     }
+    line["psnr_db"] = r["psnr_db"]
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(head, r["img"], r["blocks"])
+        line["psnr_delta_vs_oracle_db"] = round(r["psnr_db"] - line["cpu_baseline"].pop("oracle_psnr_db"), 6)
     if not args.no_extras:
         extras = {}
         for key in ("sinusoid_rgb8_4x4", "normal_rg8_4x4"):
             wl = wls[key]
             e = time_plain(enc, wl, max(10, args.steps), args.warmup)
             extras[key] = {"mtexels_s": round(e["mtexels_s"], 1), "kernel_ms": round(e["kernel_ms"], 4),
+                           "psnr_db": e["psnr_db"],
                            "valu_tflops": round(FLOPS_PER_BLOCK[wl["flops_key"]] * e["nblocks"] /
                                                 (e["kernel_ms"] * 1e-3) / 1e12, 2)}
         if not args.no_batched:
