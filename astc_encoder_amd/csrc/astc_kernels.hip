@@ -166,7 +166,7 @@ __device__ __forceinline__ void process_item_ch(const BatchArgs& a, const BatchI
 // (the front of the list) or only the 4x4 items, each with its own register
 // budget (the 4x4 kernel runs at four waves per SIMD).
 template <int SEL>
-struct BatchWaves { static constexpr int value = SEL == 4 ? 3 : 2; };
+struct BatchWaves { static constexpr int value = SEL == 4 ? 4 : 2; };
 
 template <int SEL>
 __global__ void __launch_bounds__(kPlainThreads) __attribute__((amdgpu_waves_per_eu(BatchWaves<SEL>::value)))

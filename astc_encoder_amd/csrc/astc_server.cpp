@@ -263,6 +263,21 @@ int astc_server_create(astc_ctx* ctx, const astc_server_config* cfg, astc_server
             if ((rc = ensure_capacity(sv, s, 1024, 65536, 65536)) != ASTC_OK) break;
         }
     }
+    // Replay every slot's graph once on an empty work list (the kernels exit at
+    // once), with a ring-to-ring copy and an event on the slot's stream, so that
+    // the runtime's first-use work (queue assignment, code-object and copy-kernel
+    // loading) happens here rather than under the first requests.
+    for (Slot& s : sv->slots) {
+        if (rc != ASTC_OK) break;
+        s.h_scalars[0] = s.h_scalars[1] = s.h_scalars[2] = s.h_scalars[3] = 0;
+        if ((e = hipMemcpyAsync(s.d_scalars, s.h_scalars, 4 * sizeof(int), hipMemcpyHostToDevice, s.stream)) !=
+                hipSuccess ||
+            (e = hipGraphLaunch(s.exec, s.stream)) != hipSuccess ||
+            (e = hipMemcpyAsync(sv->ring_base + 128, sv->ring_base, 128, hipMemcpyDeviceToDevice, s.stream)) !=
+                hipSuccess ||
+            (e = hipEventRecord(s.done, s.stream)) != hipSuccess || (e = hipEventSynchronize(s.done)) != hipSuccess)
+            rc = hip_fail(e, "slot warm-up");
+    }
     if (rc != ASTC_OK) {
         astc_server_destroy(sv);
         return rc;

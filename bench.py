@@ -197,6 +197,10 @@ def serving_sweep(enc, rates=(1000, 2000, 5000, 10000, 20000, 50000), seconds=0.
     rng = np.random.default_rng(0)
     warm = [(pool[i % pool_size], 0.0, None) for i in range(2 * pool_size)]
     srv.run_trace(warm)
+    # and a short Poisson stream at the top rate, so that every slot has served
+    # batches of every size before anything is timed
+    warm_t = np.cumsum(np.random.default_rng(99).exponential(1.0 / rates[-1], 5000))
+    srv.run_trace([(pool[i % pool_size], float(a), None) for i, a in enumerate(warm_t)])
     results = []
     for lam in rates:
         n = max(200, int(lam * seconds))
