@@ -71,20 +71,18 @@ __device__ __forceinline__ void store_block(uint32_t* dst, int id, const Block12
 }
 
 // Encode blocks [first, first + count) of one image with the workgroup's
-// threads: step k, thread t takes block first + k * blockDim.x + t.  With
-// prefetch the next step's texels are loaded as soon as the current ones are
-// converted (into the same registers), so the loads are in flight while the
-// block is encoded; without, after the block is stored.
+// threads: step k, thread t takes block first + k * blockDim.x + t.  `cur`
+// holds the packed texels of the thread's first block (the caller issues those
+// loads).  With prefetch the next step's texels are loaded as soon as the
+// current ones are converted (into the same registers), so the loads are in
+// flight while the block is encoded; without, after the block is stored.
 template <int DIM, bool ALPHA, int SRGB, int CH, int METHOD = METHOD_PCA>
-__device__ __forceinline__ void encode_chunk(const FetchArgs& f, int xblocks, uint32_t* dst, int first, int count,
-                                             bool prefetch, const float* lut, const PackTables* trit) {
+__device__ __forceinline__ void encode_loaded(const FetchArgs& f, int xblocks, uint32_t* dst, int first, int count,
+                                              bool prefetch, const float* lut, const PackTables* trit,
+                                              uint32_t cur[DIM * DIM]) {
     const int stride = blockDim.x;
-    int o = threadIdx.x;
-    if (o >= count) return;
-    uint32_t cur[DIM * DIM];
-    load_block_id<DIM>(f, xblocks, first + o, cur);
 #pragma unroll 1
-    for (;; o += stride) {
+    for (int o = threadIdx.x;; o += stride) {
         const bool has_next = o + stride < count;
         float x[DIM * DIM][4];
         unpack_texels<DIM, SRGB, CH>(cur, f, lut, x);
@@ -94,6 +92,15 @@ __device__ __forceinline__ void encode_chunk(const FetchArgs& f, int xblocks, ui
         if (!has_next) break;
         if (!prefetch) load_block_id<DIM>(f, xblocks, first + o + stride, cur);
     }
+}
+
+template <int DIM, bool ALPHA, int SRGB, int CH, int METHOD = METHOD_PCA>
+__device__ __forceinline__ void encode_chunk(const FetchArgs& f, int xblocks, uint32_t* dst, int first, int count,
+                                             bool prefetch, const float* lut, const PackTables* trit) {
+    if (int(threadIdx.x) >= count) return;
+    uint32_t cur[DIM * DIM];
+    load_block_id<DIM>(f, xblocks, first + threadIdx.x, cur);
+    encode_loaded<DIM, ALPHA, SRGB, CH, METHOD>(f, xblocks, dst, first, count, prefetch, lut, trit, cur);
 }
 
 // 4x4 blocks fit in 128 VGPRs (four waves per SIMD); 6x6 blocks hold 144
@@ -179,12 +186,17 @@ encode_batch_kernel(BatchArgs a) {
     const int begin = SEL == 4 ? first4 : 0;
     const int end = SEL == 6 ? first4 : nitems;
     int* counter = a.counters + (SEL == 4 ? 1 : 0);
+    // Thread 0 claims the workgroup's next item while the current one is being
+    // encoded, so the atomic's round trip is off the critical path.
+    int claimed = 0;
+    if (threadIdx.x == 0) claimed = atomicAdd(counter, 1);
     for (;;) {
         __syncthreads();  // tables staged / everyone done with the previous item index
-        if (threadIdx.x == 0) next_item = begin + atomicAdd(counter, 1);
+        if (threadIdx.x == 0) next_item = begin + claimed;
         __syncthreads();
         const int it = __builtin_amdgcn_readfirstlane(next_item);
         if (it >= end) break;
+        if (threadIdx.x == 0) claimed = atomicAdd(counter, 1);
         const BatchItem item = a.items[it];
         // one code path per block size x alpha; linear and sRGB fetches share
         // it through the table choice (separate specialisations would not fit
