@@ -17,10 +17,11 @@
 //     own rsq is an approximate hardware instruction too);
 //   * length(v) < 1e-5 is the equivalent exact test on dot(v, v);
 //   * three exact rearrangements of the shader's formulas save a quarter of
-//     the arithmetic: the power iteration runs on S = M M with its single
-//     effective length test, min/max projections are dot(texel, v) minus
-//     dot(mean, v), and grid weights rescale texel projections (dot(k, e0)
-//     cancels in the rescaling).  Each is explained where it is used.
+//     the arithmetic: after its first iteration the power iteration runs on
+//     S = M M without the length test (which can no longer succeed), min/max
+//     projections are dot(texel, v) minus dot(mean, v), and grid weights
+//     rescale texel projections (dot(k, e0) cancels in the rescaling).  Each is
+//     explained where it is used.
 // The file must be compiled with -ffp-contract=off so that nothing else fuses.
 #pragma once
 
@@ -81,7 +82,9 @@ __device__ __forceinline__ float clamp255(float x) { return fminf(fmaxf(x, 0.0f)
 // ---------------------------------------------------------------------------
 // Fetch (ASTC_Encode.hlsl:561-580 + D3D11 Load): out-of-bounds texels are zero;
 // RGB8 expands with alpha 255 and RG8 with blue 0, alpha 255; sRGB converts the
-// colour channels through the table; normal maps force blue = alpha = 255.
+// colour channels through its table, UNORM channels as binary32 c * (1/255)
+// times 255 (the hardware conversion, through a table too); normal maps force
+// blue = alpha = 255.
 // ---------------------------------------------------------------------------
 struct FetchArgs {
     const uint8_t* src;
@@ -93,8 +96,8 @@ struct FetchArgs {
 
 // OR-mask that applies the normal-map override to a packed RGBA8 texel.  Setting
 // the bytes to 255 before conversion equals forcing the floats to 255 after it
-// (ASTC_Encode.hlsl:575-579): the sRGB table maps 255 to exactly 255.0f, and
-// out-of-bounds (zero) texels get blue = alpha = 255 as well.
+// (ASTC_Encode.hlsl:575-579): the sRGB and UNORM tables both map 255 to exactly
+// 255.0f, and out-of-bounds (zero) texels get blue = alpha = 255 as well.
 __host__ __device__ inline uint32_t normal_or_mask(uint32_t flags) { return (flags & FLAG_NORMAL) ? 0xFFFF0000u : 0u; }
 
 __device__ __forceinline__ int bytes_per_texel(int fmt) { return fmt == FMT_RGBA8 ? 4 : fmt == FMT_RGB8 ? 3 : 2; }
@@ -141,19 +144,15 @@ __device__ __forceinline__ void load_row_fast(const FetchArgs& a, int x0, int y,
     }
 }
 
-// RGBA8 -> floats (x255 scale); SRGB linearises the colour channels by table.
+// RGBA8 -> floats (x255 scale) through LDS tables: `colour` converts the colour
+// channels (sRGB or UNORM), `unorm` the alpha.  Only the first NC channels are
+// converted (the others are known constants to the encoder, see encode_block).
 // The normal-map override is already in the word (FetchArgs::or_mask).
-// Only the first NC channels are converted (the others are known constants to
-// the encoder, see encode_block).
-template <bool SRGB, int NC = 4>
-__device__ __forceinline__ void unpack_texel(uint32_t p, const float* lut, float t[4]) {
+template <int NC = 4>
+__device__ __forceinline__ void unpack_texel(uint32_t p, const float* colour, const float* unorm, float t[4]) {
 #pragma unroll
-    for (int c = 0; c < NC && c < 3; ++c) t[c] = SRGB ? lut[(p >> (8 * c)) & 0xFFu] : float((p >> (8 * c)) & 0xFFu);
-    if (NC == 4) t[3] = float(p >> 24);
-    // Pin the converted floats: left to itself the compiler keeps the packed
-    // words alive next to them and re-converts, which costs registers.
-#pragma unroll
-    for (int c = 0; c < NC; ++c) asm volatile("" : "+v"(t[c]));
+    for (int c = 0; c < NC && c < 3; ++c) t[c] = colour[(p >> (8 * c)) & 0xFFu];
+    if (NC == 4) t[3] = unorm[p >> 24];
 }
 
 // Load a block's texels as packed RGBA8 words (loads only; no conversion), so
@@ -177,11 +176,11 @@ __device__ __forceinline__ void load_block(const FetchArgs& a, int bx, int by, u
     }
 }
 
-template <int DIM, bool SRGB, int NC = 4>
-__device__ __forceinline__ void unpack_block(const uint32_t p[DIM * DIM], uint32_t or_mask, const float* lut,
-                                             float x[DIM * DIM][4]) {
+template <int DIM, int NC = 4>
+__device__ __forceinline__ void unpack_block(const uint32_t p[DIM * DIM], uint32_t or_mask, const float* colour,
+                                             const float* unorm, float x[DIM * DIM][4]) {
 #pragma unroll
-    for (int k = 0; k < DIM * DIM; ++k) unpack_texel<SRGB, NC>(p[k] | or_mask, lut, x[k]);
+    for (int k = 0; k < DIM * DIM; ++k) unpack_texel<NC>(p[k] | or_mask, colour, unorm, x[k]);
 }
 
 // ---------------------------------------------------------------------------
@@ -344,32 +343,29 @@ __device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], co
         }
 
         // eigen_vector (:93-106).  One iteration is v <- normalize(M (M v)), after
-        // the test length(M v) < eps, which returns M v.  With S = M M formed once
-        // (S_ij = dot(row i, row j)) an iteration is v <- normalize(S v), and
-        // |M v|^2 = dot(v, S v) is the Rayleigh quotient of S at the unit v.  S is
-        // positive semi-definite, so that quotient never decreases from one power
-        // iterate to the next: only the test on the starting vector can succeed.
-        // Between iterations v is rescaled by a power of two (exact, and the
-        // direction is all the next product needs); the last step normalises.
+        // the test length(M v) < eps, which returns M v.  The first iteration is
+        // taken as written (M v0, its test, then M (M v0)).  Past it, the test
+        // cannot succeed: |M v|^2 = dot(v, S v) with S = M M is the Rayleigh
+        // quotient of the positive semi-definite S at the unit v, and it never
+        // decreases along power iterates.  The remaining seven iterations use S
+        // (formed once, S_ij = dot(row i, row j)), rescaling between them by a
+        // power of two (exact; only the direction feeds the next product), and
+        // the last one normalises.
         float s2[4][4];
 #pragma unroll
         for (int i = 0; i < NC; ++i)
 #pragma unroll
             for (int j = i; j < NC; ++j) s2[i][j] = s2[j][i] = dotn<NC>(m[i], m[j]);
-        v[0] = 0.26726f;
-        v[1] = 0.80178f;
-        v[2] = 0.53452f;
-        v[3] = 0.0f;
-        float w[4];
+        const float v0[4] = {0.26726f, 0.80178f, 0.53452f, 0.0f};
+        float u[4], w[4];
 #pragma unroll
-        for (int i = 0; i < NC; ++i) w[i] = dotn<NC>(s2[i], v);
-        if (is_short(dotn<NC>(v, w))) {
-            float u[4];
-#pragma unroll
-            for (int i = 0; i < NC; ++i) u[i] = dotn<NC>(m[i], v);
+        for (int i = 0; i < NC; ++i) u[i] = dotn<NC>(m[i], v0);
+        if (is_short(dotn<NC>(u, u))) {
 #pragma unroll
             for (int i = 0; i < NC; ++i) v[i] = u[i];
         } else {
+#pragma unroll
+            for (int i = 0; i < NC; ++i) w[i] = dotn<NC>(m[i], u);
 #pragma unroll
             for (int it = 1; it < 8; ++it) {
                 float mx = fabsf(w[0]);

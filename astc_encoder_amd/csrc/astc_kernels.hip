@@ -24,14 +24,15 @@ struct PlainArgs {
     const DeviceTables* tables;
 };
 
-// LDS tables: lut[0..255] sRGB -> float(linear) * 255, lut[256..511] the
-// identity (c -> float(c), the exact linear fetch), and the trit groups' code
-// bits for both weight quantisations.
+// LDS tables: lut[0..255] sRGB -> float(linear) * 255, lut[256..511] UNORM8 ->
+// float * 255 as the hardware converts it (binary32 c * (1/255), then x255; 126
+// values land one ulp off the integer), and the trit groups' code bits for both
+// weight quantisations.
 __device__ __forceinline__ void stage_tables(const DeviceTables* t, float* lut, PackTables* pack) {
     const int tid = threadIdx.x;
     for (int i = tid; i < 256; i += blockDim.x) {
         lut[i] = t->srgb[i];
-        lut[256 + i] = float(i);
+        lut[256 + i] = (float(i) * (1.0f / 255.0f)) * 255.0f;
         const uint32_t T = t->trit[i];
         pack->spread[0][i] = trit_group_spread(T, kSplitQ6.bits);
         pack->spread[1][i] = trit_group_spread(T, kSplitQ12.bits);
@@ -62,8 +63,8 @@ __host__ __device__ inline int channel_class(int fmt, uint32_t flags, int width,
 template <int DIM, int SRGB, int CH>
 __device__ __forceinline__ void unpack_texels(const uint32_t p[DIM * DIM], const FetchArgs& f, const float* lut,
                                               float x[DIM * DIM][4]) {
-    if (SRGB < 0) unpack_block<DIM, true, CH>(p, f.or_mask, lut + ((f.flags & FLAG_SRGB) ? 0 : 256), x);
-    else unpack_block<DIM, SRGB == 1, CH>(p, f.or_mask, lut, x);
+    const float* colour = SRGB < 0 ? lut + ((f.flags & FLAG_SRGB) ? 0 : 256) : SRGB == 1 ? lut : lut + 256;
+    unpack_block<DIM, CH>(p, f.or_mask, colour, lut + 256, x);
 }
 
 __device__ __forceinline__ void store_block(uint32_t* dst, int id, const Block128& b) {

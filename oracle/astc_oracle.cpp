@@ -412,11 +412,12 @@ template <typename T> void matvec(const Arith<T>& ar, const T m[4][4], const T* 
 
 /* eigen_vector in the canonical schedule.  One iteration of :97-104 is
  * v <- normalize(M (M v)) after the test length(M v) < eps (which returns M v).
- * With S = M M (S_ij = dot(row i, row j)), |M v|^2 = dot(v, S v) is the Rayleigh
- * quotient of the positive semi-definite S at the unit v, which never decreases
- * along power iterates, so only the test on the starting vector can succeed.
- * Iterates are rescaled by powers of two (exact; only the direction feeds the
- * next product) and the eighth is normalised. */
+ * The first iteration is taken as written: M v0, its test, M (M v0).  Past it
+ * the test cannot succeed: with S = M M (S_ij = dot(row i, row j)), |M v|^2 =
+ * dot(v, S v) is the Rayleigh quotient of the positive semi-definite S at the
+ * unit v, which never decreases along power iterates.  The other seven
+ * iterations multiply by S, rescaling by powers of two in between (exact; only
+ * the direction feeds the next product); the eighth product is normalised. */
 template <typename T> int exponent_of(T x) {
     int e;
     std::frexp(x, &e);
@@ -426,14 +427,13 @@ template <typename T> void eigen_canonical(const Arith<T>& ar, const T m[4][4], 
     T s2[4][4];
     for (int i = 0; i < 4; ++i)
         for (int j = i; j < 4; ++j) s2[i][j] = s2[j][i] = ar.dot(m[i], m[j]);
-    T w[4];
-    matvec(ar, s2, v, w);
-    if (short_dot(ar.dot(v, w))) {
-        T u[4];
-        matvec(ar, m, v, u);
+    T u[4], w[4];
+    matvec(ar, m, v, u);
+    if (short_vec(ar, u)) {
         std::memcpy(v, u, sizeof(u));
         return;
     }
+    matvec(ar, m, u, w);
     for (int it = 1; it < 8; ++it) {
         const T mx = std::fmax(std::fmax(std::fabs(w[0]), std::fabs(w[1])), std::fmax(std::fabs(w[2]), std::fabs(w[3])));
         const int e = exponent_of(mx);
@@ -649,6 +649,21 @@ struct SrgbLut {
 };
 const SrgbLut& srgb_lut() { static SrgbLut l; return l; }
 
+/* UNORM8 -> float as D3D hardware converts it, a binary32 multiply by the
+ * rounded reciprocal of 255, then the shader's x255 (:579): 126 of the 256
+ * values land one ulp off the integer.  (A correctly rounded c / 255 would give
+ * every integer back exactly; the reference's golden output shows the rounding
+ * noise of an inexact conversion -- blocks whose exact covariance has no
+ * colour-alpha coupling still found their alpha axis -- see the leaf tests.)
+ * The binary64 twin keeps the ideal value c. */
+struct UnormLut {
+    float f[256];
+    UnormLut() {
+        for (int c = 0; c < 256; ++c) f[c] = ((float)c * (1.0f / 255.0f)) * 255.0f;
+    }
+};
+const UnormLut& unorm_lut() { static UnormLut l; return l; }
+
 /* RGBA8 of texel (x, y) with D3D's expansion of narrower formats:
  * RGB8 -> alpha 255; RG8 -> blue 0, alpha 255. */
 void load_rgba8(const uint8_t* src, int row_pitch, int fmt, int x, int y, uint8_t rgba[4]) {
@@ -670,7 +685,7 @@ void fetch_block(const uint8_t* src, int width, int height, int row_pitch, int f
             load_rgba8(src, row_pitch, fmt, x, y, p);
             for (int c = 0; c < 4; ++c) {
                 if (srgb && c < 3) t[c] = std::is_same<T, float>::value ? (T)srgb_lut().f[p[c]] : (T)srgb_lut().d[p[c]];
-                else t[c] = (T)p[c];                            /* float(c/255) * 255 == c exactly */
+                else t[c] = std::is_same<T, float>::value ? (T)unorm_lut().f[p[c]] : (T)p[c];
             }
         }
         if (normal) { t[2] = T(255); t[3] = T(255); }          /* :575-578, then x255 */
@@ -974,6 +989,7 @@ int64_t oracle_encode_image(const uint8_t* src, int width, int height, int row_p
 }
 
 void oracle_srgb_table(float out[256]) { std::memcpy(out, srgb_lut().f, sizeof(float) * 256); }
+void oracle_unorm_table(float out[256]) { std::memcpy(out, unorm_lut().f, sizeof(float) * 256); }
 
 void oracle_tables(int32_t trits_out[243], int32_t quints_out[125], int32_t scramble_out[12 * 32],
                    int32_t btq_out[21 * 3]) {

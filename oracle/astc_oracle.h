@@ -66,7 +66,9 @@ int oracle_encode_block_method_f64(const double* texels, int dim, int has_alpha,
                                    uint8_t out[16], oracle_block_trace* trace);
 
 /* Fetch one block's texels exactly as MainCS does (ASTC_Encode.hlsl:561-580):
- * out-of-bounds texels are zero, normal maps force b = a = 1, then x255. */
+ * out-of-bounds texels are zero, normal maps force b = a = 1, then x255.
+ * UNORM channels convert as binary32 c * (1/255) (the hardware conversion)
+ * before the x255; sRGB colour channels through oracle_srgb_table. */
 void oracle_fetch_block_f32(const uint8_t* src, int width, int height, int row_pitch, int src_format,
                             int dim, int flags, int bx, int by, float* texels);
 void oracle_fetch_block_f64(const uint8_t* src, int width, int height, int row_pitch, int src_format,
@@ -79,6 +81,8 @@ int64_t oracle_encode_image(const uint8_t* src, int width, int height, int row_p
 
 /* sRGB -> linear x255 table used for SRGB fetches (binary32). */
 void oracle_srgb_table(float out[256]);
+/* UNORM8 -> float x255 values of the binary32 fetch. */
+void oracle_unorm_table(float out[256]);
 
 /* ---- ISE and block assembly restatement (ASTC_IntegerSequenceEncoding.hlsl) ---- */
 void oracle_tables(int32_t trits_out[243], int32_t quints_out[125], int32_t scramble_out[12 * 32],

@@ -14,6 +14,13 @@ that has the reference mounted, and writes small data files:
   leaf_crop.astc    the reference blocks covering that crop, with the .astc
                     header, i.e. what `astc_cs_enc leaf_crop.png -alpha -4x4`
                     must reproduce.
+  leaf_uncoupled_alpha.npz
+                    every block of the image whose tile has exactly zero
+                    covariance between alpha and each colour channel (integer
+                    arithmetic) and an alpha variance above every colour
+                    eigenvalue: its tile and the reference's block.  In exact
+                    arithmetic the shader's power iteration, which starts with
+                    no alpha component, cannot reach the alpha axis there.
   leaf_stats.json   whole-image statistics of the reference output.
 
 Usage: python tests/golden/make_leaf_fixture.py [/root/reference]
@@ -71,6 +78,18 @@ def main(ref_root):
     with open(os.path.join(HERE, "leaf_crop.astc"), "wb") as f:
         f.write(hdr + blocks[sel].tobytes())
 
+    # tiles with no colour-alpha coupling and a dominant alpha variance
+    all_tiles = flipped.reshape(h // 4, 4, xb, 4, 4).transpose(0, 2, 1, 3, 4).reshape(-1, 16, 4).astype(np.int64)
+    s1 = all_tiles.sum(1)                                         # per channel sums
+    cross = 16 * np.einsum("nkc,nk->nc", all_tiles[:, :, :3], all_tiles[:, :, 3]) - s1[:, :3] * s1[:, 3:4]
+    uncoupled = np.all(cross == 0, axis=1)
+    d = all_tiles - all_tiles.mean(1, keepdims=True)
+    cov = np.einsum("nki,nkj->nij", d, d) / 15.0
+    top_colour = np.linalg.eigvalsh(cov[:, :3, :3])[:, -1]
+    pick = np.nonzero(uncoupled & (cov[:, 3, 3] > top_colour))[0]
+    np.savez_compressed(os.path.join(HERE, "leaf_uncoupled_alpha.npz"), ids=pick.astype(np.int32),
+                        tiles=all_tiles[pick].reshape(-1, 4, 4, 4).astype(np.uint8), blocks=blocks[pick])
+
     modes = blocks[:, 0].astype(np.uint32) | (blocks[:, 1].astype(np.uint32) << 8)
     stats = {
         "source": "niepp/astc_encoder textures/leaf.png + textures/leaf.astc (MIT)",
@@ -79,6 +98,7 @@ def main(ref_root):
         "header_hex": header.tobytes().hex(),
         "blockmode_0x043_fraction": float(np.mean((modes & 0x7FF) == 0x043)),
         "sample_block_ids_seed": 0,
+        "uncoupled_alpha_blocks": int(pick.size),
     }
     with open(os.path.join(HERE, "leaf_stats.json"), "w") as f:
         json.dump(stats, f, indent=1)

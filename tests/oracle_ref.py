@@ -48,6 +48,7 @@ def lib():
             "oracle_fetch_block_f64": (None, [P, I, I, I, I, I, I, I, I, P]),
             "oracle_encode_image": (ctypes.c_int64, [P, I, I, I, I, I, I, I, I, P]),
             "oracle_srgb_table": (None, [P]),
+            "oracle_unorm_table": (None, [P]),
             "oracle_tables": (None, [P, P, P, P]),
             "oracle_bise": (I, [P, I, I, P]),
             "oracle_blockmode": (ctypes.c_uint32, [I]),
@@ -170,4 +171,10 @@ def pack_block(ep_q, w_q, has_alpha):
 def srgb_table():
     out = np.zeros(256, np.float32)
     lib().oracle_srgb_table(_ptr(out))
+    return out
+
+
+def unorm_table():
+    out = np.zeros(256, np.float32)
+    lib().oracle_unorm_table(_ptr(out))
     return out

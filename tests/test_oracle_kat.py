@@ -129,9 +129,30 @@ def test_srgb_table_endpoints_and_monotone():
     c = np.arange(256) / 255.0
     lin = np.where(c <= 0.04045, c / 12.92, ((c + 0.055) / 1.055) ** 2.4) * 255.0
     assert np.max(np.abs(lut - lin)) < 1e-4
-    # the linear (UNORM) fetch is exact: float(c/255) * 255 == c for every byte
+    # a correctly rounded c / 255 would give every byte back exactly after x255
     u = (np.arange(256, dtype=np.float32) / np.float32(255.0)).astype(np.float32) * np.float32(255.0)
     assert np.array_equal(u, np.arange(256, dtype=np.float32))
+
+
+def test_unorm_fetch_models_the_hardware_conversion():
+    """UNORM8 texels convert as binary32 c * fl(1/255), then the shader's x255:
+    within one ulp of c, exact for 0 and 255 (so constant-255 channels stay exact)."""
+    f = np.float32
+    c = np.arange(256, dtype=np.float32)
+    want = ((c * (f(1) / f(255))).astype(f) * f(255)).astype(f)
+    got = orc.unorm_table()
+    np.testing.assert_array_equal(got, want)
+    assert got[0] == 0 and got[255] == 255
+    off = got != c
+    assert off.sum() == 126
+    assert np.all(np.abs(got[off] - c[off]) <= np.spacing(c[off]))
+    # the fetch uses it for UNORM channels, the sRGB table for sRGB colour
+    img = np.arange(64, dtype=np.uint8).reshape(4, 4, 4) * 3
+    t = orc.fetch_block(img, 4, 0, 0, 0)
+    np.testing.assert_array_equal(t, got[img.reshape(16, 4)])
+    ts = orc.fetch_block(img, 4, 4, 0, 0)  # sRGB: colour by table, alpha UNORM
+    np.testing.assert_array_equal(ts[:, :3], orc.srgb_table()[img.reshape(16, 4)[:, :3]])
+    np.testing.assert_array_equal(ts[:, 3], got[img.reshape(16, 4)[:, 3]])
 
 
 @pytest.mark.parametrize("dim,alpha", [(4, False), (4, True), (6, True), (6, False)])

@@ -6,11 +6,16 @@ linear) and the reference's blocks for them (-alpha -4x4).
 
 The reference ran on an unknown D3D11 GPU with FXC code generation, so its
 floating-point rounding is not reproducible; blocks whose weights sit on exact
-x.5 ties in exact arithmetic can round either way.  Over the whole image the
-oracle matches 98.16% of the blocks byte for byte and its PSNR is 0.047 dB
-below the reference's (38.206 vs 38.254 dB).  What the reference fixes exactly
-is checked exactly: decoding each reference block and re-packing its endpoints
-and weights with the oracle reproduces the block bit for bit.
+x.5 ties in exact arithmetic can round either way.  Its fetch was not exact
+either: tests/golden/leaf_uncoupled_alpha.npz holds the blocks whose exact
+covariance has no colour-alpha coupling while alpha dominates, where exact
+arithmetic can never reach the alpha axis, yet the reference did in 112 of the
+122 that vary in colour.  The oracle models the UNORM conversion as binary32
+c * fl(1/255); over the whole image it then matches 97.5% of the blocks byte for
+byte and its PSNR is 0.002 dB below the reference's (38.252 vs 38.254 dB; an exact
+conversion gives 97.8% and 0.046 dB below).  What the reference fixes exactly is
+checked exactly: decoding each reference block and re-packing its endpoints and
+weights with the oracle reproduces the block bit for bit.
 """
 import os
 
@@ -48,7 +53,7 @@ def test_repack_of_reference_blocks_is_bit_exact(leaf_sample):
 
 def test_oracle_matches_reference_blocks(leaf_sample):
     img = sample_image(leaf_sample)
-    for sched, min_match in ((orc.SCHED_CANONICAL, 0.98), (orc.SCHED_SOURCE, 0.98)):
+    for sched, min_match in ((orc.SCHED_CANONICAL, 0.97), (orc.SCHED_SOURCE, 0.97)):
         out = orc.oracle_encode(img, 4, 1, sched)
         match = np.mean(np.all(out == leaf_sample["blocks"], axis=1))
         assert match >= min_match, (sched, match)
@@ -64,7 +69,7 @@ def test_oracle_psnr_within_tolerance_of_reference(leaf_sample):
         ok, dec = orc.decode_image(out, w, h, 4)
         assert ok
         delta = orc.psnr(img, 1, dec) - orc.psnr(img, 1, dec_ref)
-        assert abs(delta) <= 0.05, (sched, delta)
+        assert abs(delta) <= 0.01, (sched, delta)
 
 
 def test_mismatches_are_weight_ties(leaf_sample):
@@ -89,3 +94,34 @@ def test_crop_fixture_header(golden_dir):
     data = np.fromfile(os.path.join(golden_dir, "leaf_crop.astc"), np.uint8)
     assert data[:4].tobytes() == bytes([0x13, 0xAB, 0xA1, 0x5C])
     assert data.size == 16 + 32 * 32 * 16
+
+
+def alpha_split(blocks):
+    return np.array([orc.decode_block(b, 4)[2].ep_q[6] != orc.decode_block(b, 4)[2].ep_q[7] for b in blocks])
+
+
+def test_uncoupled_alpha_blocks_pin_an_inexact_fetch(golden_dir):
+    """Exact arithmetic keeps the power iteration's alpha component at zero on these
+    tiles (no colour-alpha coupling, start vector without alpha), so the encoder
+    collapses alpha to its mean; the reference split alpha on most that vary in
+    colour, which only rounding noise in its fetch explains.  The oracle's fetch
+    model reproduces part of that, and only where the reference did it."""
+    u = np.load(os.path.join(golden_dir, "leaf_uncoupled_alpha.npz"))
+    tiles, blocks = u["tiles"], u["blocks"]
+    n = len(tiles)
+    flat = tiles.reshape(n, 16, 4).astype(np.int64)
+    # the fixture's defining property, recomputed in integers
+    cross = 16 * np.einsum("nkc,nk->nc", flat[:, :, :3], flat[:, :, 3]) - flat[:, :, :3].sum(1) * flat[:, :, 3:].sum(1)
+    assert np.all(cross == 0)
+    colour_varies = ~np.all(flat[:, :, :3] == flat[:, :1, :3], axis=(1, 2))
+    assert colour_varies.sum() == 122
+    ref = alpha_split(blocks)
+    assert not np.any(ref[~colour_varies])            # alpha-only variance collapses (M v0 = 0)
+    assert ref[colour_varies].sum() >= 110
+    for t in tiles[colour_varies]:                     # binary64, exact texel values
+        _, tr = orc.encode_block(t.reshape(16, 4).astype(np.float64), 4, True, orc.SCHED_CANONICAL, f64=True)
+        assert tr.ep_q[6] == tr.ep_q[7] and tr.axis[3] == 0.0
+    img = np.concatenate(list(tiles), axis=1)          # the tiles side by side, one block each
+    ours = alpha_split(orc.oracle_encode(img, 4, 1))
+    assert not np.any(ours & ~ref)
+    assert ours[colour_varies].sum() >= 40
