@@ -260,29 +260,34 @@ int astc_batch_autotune(astc_batch* b, void* stream, int reps, astc_tuning* best
     }
     astc_tuning best = b->tuning;
     float best_ms = 1e30f;
-    int rc = ASTC_OK;
+    std::vector<astc_tuning> candidates;
     for (int split : {0, 1})
-    for (int bpl : {1, 2, 4, 8})
-        for (int wpc : {1, 2})
-            for (int pf : {0, 1}) {
-                if (bpl == 1 && pf == 1) continue;  // nothing to prefetch within an item
-                astc_tuning t{bpl, wpc, pf, split};
-                if ((rc = astc_batch_set_tuning(b, &t)) != ASTC_OK) break;
-                if ((rc = astc_batch_launch(b, stream)) != ASTC_OK) break;  // warm-up
-                (void)hipEventRecord(t0, s);
-                for (int r = 0; r < reps && rc == ASTC_OK; ++r) rc = astc_batch_launch(b, stream);
-                (void)hipEventRecord(t1, s);
-                if ((e = hipEventSynchronize(t1)) != hipSuccess) {
-                    rc = hip_fail(e, "hipEventSynchronize");
-                    break;
-                }
-                float ms = 0.f;
-                (void)hipEventElapsedTime(&ms, t0, t1);
-                if (ms < best_ms) {
-                    best_ms = ms;
-                    best = t;
-                }
-            }
+        for (int bpl : {1, 2, 4, 8})
+            for (int wpc : {1, 2})
+                for (int pf : {0, 1})
+                    if (!(bpl == 1 && pf == 1))  // nothing to prefetch within an item
+                        candidates.push_back(astc_tuning{bpl, wpc, pf, split});
+    int rc = ASTC_OK;
+    for (const astc_tuning& t : candidates) {
+        if ((rc = astc_batch_set_tuning(b, &t)) != ASTC_OK) break;
+        if ((rc = astc_batch_launch(b, stream)) != ASTC_OK) break;  // warm-up
+        if ((e = hipEventRecord(t0, s)) != hipSuccess) {
+            rc = hip_fail(e, "hipEventRecord");
+            break;
+        }
+        for (int r = 0; r < reps && rc == ASTC_OK; ++r) rc = astc_batch_launch(b, stream);
+        if (rc != ASTC_OK) break;
+        if ((e = hipEventRecord(t1, s)) != hipSuccess || (e = hipEventSynchronize(t1)) != hipSuccess) {
+            rc = hip_fail(e, "timing event");
+            break;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, t0, t1);
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = t;
+        }
+    }
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     if (rc != ASTC_OK) return rc;
