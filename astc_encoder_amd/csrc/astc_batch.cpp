@@ -160,76 +160,80 @@ extern "C" {
 
 int astc_batch_create(astc_ctx* ctx, const astc_image_desc* descs, const void* const* srcs, void* const* dsts,
                       int count, astc_batch** out) {
-    if (!out) return fail(ASTC_ERR_INVALID_ARG, "out is NULL");
-    *out = nullptr;
-    if (!ctx || !descs || !srcs || !dsts || count < 1) return fail(ASTC_ERR_INVALID_ARG, "empty or NULL batch");
-    hipError_t e = hipSetDevice(ctx->device);
-    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-    astc_batch* b = new (std::nothrow) astc_batch();
-    if (!b) return fail(ASTC_ERR_NO_MEMORY, "host allocation failed");
-    b->ctx = ctx;
-    b->device = ctx->device;
-    b->levels.resize(count);
-    b->modes.resize(count);
-    for (int i = 0; i < count; ++i) {
-        ImageGeom g;
-        int rc = check_desc(&descs[i], &g);
-        if (rc == ASTC_OK && (descs[i].flags & ASTC_FLAG_MAX_ACCUMULATION))
-            rc = fail(ASTC_ERR_UNSUPPORTED, "the max-accumulation search is available in astc_encode only");
-        if (rc != ASTC_OK || !srcs[i] || !dsts[i] || (reinterpret_cast<uintptr_t>(dsts[i]) & 15u)) {
-            delete b;
-            if (rc == ASTC_OK) rc = fail(ASTC_ERR_INVALID_ARG, "image " + std::to_string(i) + ": bad src/dst pointer");
-            else set_error("image " + std::to_string(i) + ": " + astc_last_error());
+    return guarded([&]() -> int {
+        if (!out) return fail(ASTC_ERR_INVALID_ARG, "out is NULL");
+        *out = nullptr;
+        if (!ctx || !descs || !srcs || !dsts || count < 1) return fail(ASTC_ERR_INVALID_ARG, "empty or NULL batch");
+        hipError_t e = hipSetDevice(ctx->device);
+        if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+        astc_batch* b = new (std::nothrow) astc_batch();
+        if (!b) return fail(ASTC_ERR_NO_MEMORY, "host allocation failed");
+        b->ctx = ctx;
+        b->device = ctx->device;
+        b->levels.resize(count);
+        b->modes.resize(count);
+        for (int i = 0; i < count; ++i) {
+            ImageGeom g;
+            int rc = check_desc(&descs[i], &g);
+            if (rc == ASTC_OK && (descs[i].flags & ASTC_FLAG_MAX_ACCUMULATION))
+                rc = fail(ASTC_ERR_UNSUPPORTED, "the max-accumulation search is available in astc_encode only");
+            if (rc != ASTC_OK || !srcs[i] || !dsts[i] || (reinterpret_cast<uintptr_t>(dsts[i]) & 15u)) {
+                delete b;
+                if (rc == ASTC_OK) rc = fail(ASTC_ERR_INVALID_ARG, "image " + std::to_string(i) + ": bad src/dst pointer");
+                else set_error("image " + std::to_string(i) + ": " + astc_last_error());
+                return rc;
+            }
+            BatchLevel& L = b->levels[i];
+            L.f.src = static_cast<const uint8_t*>(srcs[i]);
+            L.f.width = descs[i].width;
+            L.f.height = descs[i].height;
+            L.f.row_pitch = descs[i].row_pitch;
+            L.f.fmt = descs[i].src_format;
+            L.f.flags = descs[i].flags;
+            L.f.aligned = ((reinterpret_cast<uintptr_t>(srcs[i]) & 3u) == 0) && ((descs[i].row_pitch & 3) == 0);
+            L.f.or_mask = normal_or_mask(descs[i].flags);
+            L.dst = static_cast<uint32_t*>(dsts[i]);
+            L.xblocks = g.xblocks;
+            L.nblocks = (int)g.nblocks;
+            b->modes[i] = mode_of(descs[i].block_dim, descs[i].src_format, descs[i].flags, descs[i].width, descs[i].height);
+            b->blocks += g.nblocks;
+            b->texels += (int64_t)descs[i].width * descs[i].height;
+        }
+        int rc = ASTC_OK;
+        if ((e = hipDeviceGetAttribute(&b->num_cus, hipDeviceAttributeMultiprocessorCount, ctx->device)) != hipSuccess)
+            rc = hip_fail(e, "hipDeviceGetAttribute");
+        else if ((e = hipMalloc(&b->d_levels, count * sizeof(BatchLevel))) != hipSuccess)
+            rc = hip_fail(e, "hipMalloc(levels)");
+        else if ((e = hipMemcpy(b->d_levels, b->levels.data(), count * sizeof(BatchLevel), hipMemcpyHostToDevice)) !=
+                 hipSuccess)
+            rc = hip_fail(e, "hipMemcpy(levels)");
+        else if ((e = hipMalloc(&b->d_scalars, 4 * sizeof(int))) != hipSuccess)
+            rc = hip_fail(e, "hipMalloc(scalars)");
+        else if ((e = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking)) != hipSuccess ||
+                 (e = hipStreamCreateWithFlags(&b->side_stream, hipStreamNonBlocking)) != hipSuccess ||
+                 (e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming)) != hipSuccess ||
+                 (e = hipEventCreateWithFlags(&b->join, hipEventDisableTiming)) != hipSuccess)
+            rc = hip_fail(e, "stream/event creation");
+        else
+            rc = build(b);
+        if (rc != ASTC_OK) {
+            astc_batch_destroy(b);
             return rc;
         }
-        BatchLevel& L = b->levels[i];
-        L.f.src = static_cast<const uint8_t*>(srcs[i]);
-        L.f.width = descs[i].width;
-        L.f.height = descs[i].height;
-        L.f.row_pitch = descs[i].row_pitch;
-        L.f.fmt = descs[i].src_format;
-        L.f.flags = descs[i].flags;
-        L.f.aligned = ((reinterpret_cast<uintptr_t>(srcs[i]) & 3u) == 0) && ((descs[i].row_pitch & 3) == 0);
-        L.f.or_mask = normal_or_mask(descs[i].flags);
-        L.dst = static_cast<uint32_t*>(dsts[i]);
-        L.xblocks = g.xblocks;
-        L.nblocks = (int)g.nblocks;
-        b->modes[i] = mode_of(descs[i].block_dim, descs[i].src_format, descs[i].flags, descs[i].width, descs[i].height);
-        b->blocks += g.nblocks;
-        b->texels += (int64_t)descs[i].width * descs[i].height;
-    }
-    int rc = ASTC_OK;
-    if ((e = hipDeviceGetAttribute(&b->num_cus, hipDeviceAttributeMultiprocessorCount, ctx->device)) != hipSuccess)
-        rc = hip_fail(e, "hipDeviceGetAttribute");
-    else if ((e = hipMalloc(&b->d_levels, count * sizeof(BatchLevel))) != hipSuccess)
-        rc = hip_fail(e, "hipMalloc(levels)");
-    else if ((e = hipMemcpy(b->d_levels, b->levels.data(), count * sizeof(BatchLevel), hipMemcpyHostToDevice)) !=
-             hipSuccess)
-        rc = hip_fail(e, "hipMemcpy(levels)");
-    else if ((e = hipMalloc(&b->d_scalars, 4 * sizeof(int))) != hipSuccess)
-        rc = hip_fail(e, "hipMalloc(scalars)");
-    else if ((e = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking)) != hipSuccess ||
-             (e = hipStreamCreateWithFlags(&b->side_stream, hipStreamNonBlocking)) != hipSuccess ||
-             (e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming)) != hipSuccess ||
-             (e = hipEventCreateWithFlags(&b->join, hipEventDisableTiming)) != hipSuccess)
-        rc = hip_fail(e, "stream/event creation");
-    else
-        rc = build(b);
-    if (rc != ASTC_OK) {
-        astc_batch_destroy(b);
-        return rc;
-    }
-    *out = b;
-    return ASTC_OK;
+        *out = b;
+        return ASTC_OK;
+    });
 }
 
 int astc_batch_set_tuning(astc_batch* b, const astc_tuning* t) {
-    if (!b || !t) return fail(ASTC_ERR_INVALID_ARG, "NULL argument");
-    if (!valid_tuning(t)) return fail(ASTC_ERR_INVALID_ARG, "tuning out of range");
-    hipError_t e = hipSetDevice(b->device);
-    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-    b->tuning = *t;
-    return build(b);
+    return guarded([&]() -> int {
+        if (!b || !t) return fail(ASTC_ERR_INVALID_ARG, "NULL argument");
+        if (!valid_tuning(t)) return fail(ASTC_ERR_INVALID_ARG, "tuning out of range");
+        hipError_t e = hipSetDevice(b->device);
+        if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+        b->tuning = *t;
+        return build(b);
+    });
 }
 
 int astc_batch_get_tuning(const astc_batch* b, astc_tuning* t) {
@@ -248,52 +252,54 @@ int astc_batch_launch(astc_batch* b, void* stream) {
 }
 
 int astc_batch_autotune(astc_batch* b, void* stream, int reps, astc_tuning* best_out) {
-    if (!b) return fail(ASTC_ERR_INVALID_ARG, "batch is NULL");
-    if (reps < 1) reps = 1;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    hipEvent_t t0, t1;
-    hipError_t e;
-    if ((e = hipEventCreate(&t0)) != hipSuccess) return hip_fail(e, "hipEventCreate");
-    if ((e = hipEventCreate(&t1)) != hipSuccess) {
+    return guarded([&]() -> int {
+        if (!b) return fail(ASTC_ERR_INVALID_ARG, "batch is NULL");
+        if (reps < 1) reps = 1;
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        hipEvent_t t0, t1;
+        hipError_t e;
+        if ((e = hipEventCreate(&t0)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+        if ((e = hipEventCreate(&t1)) != hipSuccess) {
+            (void)hipEventDestroy(t0);
+            return hip_fail(e, "hipEventCreate");
+        }
+        astc_tuning best = b->tuning;
+        float best_ms = 1e30f;
+        std::vector<astc_tuning> candidates;
+        for (int split : {0, 1})
+            for (int bpl : {1, 2, 4, 8})
+                for (int wpc : {1, 2})
+                    for (int pf : {0, 1})
+                        if (!(bpl == 1 && pf == 1))  // nothing to prefetch within an item
+                            candidates.push_back(astc_tuning{bpl, wpc, pf, split});
+        int rc = ASTC_OK;
+        for (const astc_tuning& t : candidates) {
+            if ((rc = astc_batch_set_tuning(b, &t)) != ASTC_OK) break;
+            if ((rc = astc_batch_launch(b, stream)) != ASTC_OK) break;  // warm-up
+            if ((e = hipEventRecord(t0, s)) != hipSuccess) {
+                rc = hip_fail(e, "hipEventRecord");
+                break;
+            }
+            for (int r = 0; r < reps && rc == ASTC_OK; ++r) rc = astc_batch_launch(b, stream);
+            if (rc != ASTC_OK) break;
+            if ((e = hipEventRecord(t1, s)) != hipSuccess || (e = hipEventSynchronize(t1)) != hipSuccess) {
+                rc = hip_fail(e, "timing event");
+                break;
+            }
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, t0, t1);
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = t;
+            }
+        }
         (void)hipEventDestroy(t0);
-        return hip_fail(e, "hipEventCreate");
-    }
-    astc_tuning best = b->tuning;
-    float best_ms = 1e30f;
-    std::vector<astc_tuning> candidates;
-    for (int split : {0, 1})
-        for (int bpl : {1, 2, 4, 8})
-            for (int wpc : {1, 2})
-                for (int pf : {0, 1})
-                    if (!(bpl == 1 && pf == 1))  // nothing to prefetch within an item
-                        candidates.push_back(astc_tuning{bpl, wpc, pf, split});
-    int rc = ASTC_OK;
-    for (const astc_tuning& t : candidates) {
-        if ((rc = astc_batch_set_tuning(b, &t)) != ASTC_OK) break;
-        if ((rc = astc_batch_launch(b, stream)) != ASTC_OK) break;  // warm-up
-        if ((e = hipEventRecord(t0, s)) != hipSuccess) {
-            rc = hip_fail(e, "hipEventRecord");
-            break;
-        }
-        for (int r = 0; r < reps && rc == ASTC_OK; ++r) rc = astc_batch_launch(b, stream);
-        if (rc != ASTC_OK) break;
-        if ((e = hipEventRecord(t1, s)) != hipSuccess || (e = hipEventSynchronize(t1)) != hipSuccess) {
-            rc = hip_fail(e, "timing event");
-            break;
-        }
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, t0, t1);
-        if (ms < best_ms) {
-            best_ms = ms;
-            best = t;
-        }
-    }
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
-    if (rc != ASTC_OK) return rc;
-    if ((rc = astc_batch_set_tuning(b, &best)) != ASTC_OK) return rc;
-    if (best_out) *best_out = best;
-    return ASTC_OK;
+        (void)hipEventDestroy(t1);
+        if (rc != ASTC_OK) return rc;
+        if ((rc = astc_batch_set_tuning(b, &best)) != ASTC_OK) return rc;
+        if (best_out) *best_out = best;
+        return ASTC_OK;
+    });
 }
 
 int astc_batch_info(const astc_batch* b, int64_t* blocks, int64_t* texels, int32_t* items) {

@@ -155,68 +155,74 @@ int astc_decode(astc_ctx* ctx, int width, int height, int block_dim_x, int block
 
 int astc_encode_each(astc_ctx* ctx, const astc_image_desc* descs, const void* const* srcs, void* const* dsts,
                      int count, void* stream) {
-    if (!descs || !srcs || !dsts || count < 0) return fail(ASTC_ERR_INVALID_ARG, "NULL arrays");
-    for (int i = 0; i < count; ++i) {
-        int rc = astc_encode(ctx, &descs[i], srcs[i], dsts[i], stream);
-        if (rc != ASTC_OK) return rc;
-    }
-    return ASTC_OK;
+    return guarded([&]() -> int {
+        if (!descs || !srcs || !dsts || count < 0) return fail(ASTC_ERR_INVALID_ARG, "NULL arrays");
+        for (int i = 0; i < count; ++i) {
+            int rc = astc_encode(ctx, &descs[i], srcs[i], dsts[i], stream);
+            if (rc != ASTC_OK) return rc;
+        }
+        return ASTC_OK;
+    });
 }
 
 int astc_write_file(const char* path, int block_dim_x, int block_dim_y, int width, int height,
                     const void* blocks_host, int64_t nbytes) {
-    if (!path || !blocks_host || nbytes < 0) return fail(ASTC_ERR_INVALID_ARG, "path/blocks invalid");
-    if (block_dim_x < 4 || block_dim_x > 12 || block_dim_y < 4 || block_dim_y > 12)
-        return fail(ASTC_ERR_INVALID_ARG, "block dims must be 4..12");
-    if (width < 1 || height < 1 || width > (1 << 24) - 1 || height > (1 << 24) - 1)
-        return fail(ASTC_ERR_INVALID_ARG, "width/height must be in 1..16777215");
-    const int64_t expect = 16LL * ((width + block_dim_x - 1) / block_dim_x) * ((height + block_dim_y - 1) / block_dim_y);
-    if (nbytes != expect) return fail(ASTC_ERR_INVALID_ARG, "nbytes does not match the block count");
-    uint8_t hdr[kAstcHeaderBytes];
-    for (int i = 0; i < 4; ++i) hdr[i] = uint8_t(kAstcMagic >> (8 * i));
-    hdr[4] = uint8_t(block_dim_x);
-    hdr[5] = uint8_t(block_dim_y);
-    hdr[6] = 1;
-    for (int i = 0; i < 3; ++i) {
-        hdr[7 + i] = uint8_t(width >> (8 * i));
-        hdr[10 + i] = uint8_t(height >> (8 * i));
-    }
-    hdr[13] = 1;
-    hdr[14] = 0;
-    hdr[15] = 0;
-    FILE* f = std::fopen(path, "wb");
-    if (!f) return fail(ASTC_ERR_IO, std::string("cannot open for writing: ") + path);
-    bool ok = std::fwrite(hdr, 1, sizeof(hdr), f) == sizeof(hdr) &&
-              std::fwrite(blocks_host, 1, (size_t)nbytes, f) == (size_t)nbytes;
-    ok = (std::fclose(f) == 0) && ok;
-    if (!ok) return fail(ASTC_ERR_IO, std::string("short write: ") + path);
-    return ASTC_OK;
+    return guarded([&]() -> int {
+        if (!path || !blocks_host || nbytes < 0) return fail(ASTC_ERR_INVALID_ARG, "path/blocks invalid");
+        if (block_dim_x < 4 || block_dim_x > 12 || block_dim_y < 4 || block_dim_y > 12)
+            return fail(ASTC_ERR_INVALID_ARG, "block dims must be 4..12");
+        if (width < 1 || height < 1 || width > (1 << 24) - 1 || height > (1 << 24) - 1)
+            return fail(ASTC_ERR_INVALID_ARG, "width/height must be in 1..16777215");
+        const int64_t expect = 16LL * ((width + block_dim_x - 1) / block_dim_x) * ((height + block_dim_y - 1) / block_dim_y);
+        if (nbytes != expect) return fail(ASTC_ERR_INVALID_ARG, "nbytes does not match the block count");
+        uint8_t hdr[kAstcHeaderBytes];
+        for (int i = 0; i < 4; ++i) hdr[i] = uint8_t(kAstcMagic >> (8 * i));
+        hdr[4] = uint8_t(block_dim_x);
+        hdr[5] = uint8_t(block_dim_y);
+        hdr[6] = 1;
+        for (int i = 0; i < 3; ++i) {
+            hdr[7 + i] = uint8_t(width >> (8 * i));
+            hdr[10 + i] = uint8_t(height >> (8 * i));
+        }
+        hdr[13] = 1;
+        hdr[14] = 0;
+        hdr[15] = 0;
+        FILE* f = std::fopen(path, "wb");
+        if (!f) return fail(ASTC_ERR_IO, std::string("cannot open for writing: ") + path);
+        bool ok = std::fwrite(hdr, 1, sizeof(hdr), f) == sizeof(hdr) &&
+                  std::fwrite(blocks_host, 1, (size_t)nbytes, f) == (size_t)nbytes;
+        ok = (std::fclose(f) == 0) && ok;
+        if (!ok) return fail(ASTC_ERR_IO, std::string("short write: ") + path);
+        return ASTC_OK;
+    });
 }
 
 int astc_read_file_header(const char* path, int* block_dim_x, int* block_dim_y, int* width, int* height,
                           int64_t* payload_bytes) {
-    if (!path) return fail(ASTC_ERR_INVALID_ARG, "path is NULL");
-    FILE* f = std::fopen(path, "rb");
-    if (!f) return fail(ASTC_ERR_IO, std::string("cannot open: ") + path);
-    uint8_t h[kAstcHeaderBytes];
-    const size_t got = std::fread(h, 1, sizeof(h), f);
-    std::fseek(f, 0, SEEK_END);
-    const long size = std::ftell(f);
-    std::fclose(f);
-    if (got != sizeof(h)) return fail(ASTC_ERR_IO, "file shorter than the .astc header");
-    uint32_t magic = uint32_t(h[0]) | uint32_t(h[1]) << 8 | uint32_t(h[2]) << 16 | uint32_t(h[3]) << 24;
-    if (magic != kAstcMagic) return fail(ASTC_ERR_IO, "bad .astc magic");
-    const int bx = h[4], by = h[5];
-    const int w = h[7] | h[8] << 8 | h[9] << 16, ht = h[10] | h[11] << 8 | h[12] << 16;
-    if (bx < 4 || by < 4 || w < 1 || ht < 1) return fail(ASTC_ERR_IO, "bad .astc header fields");
-    const int64_t expect = 16LL * ((w + bx - 1) / bx) * ((ht + by - 1) / by);
-    if (size - (long)sizeof(h) != expect) return fail(ASTC_ERR_IO, "payload size does not match the header");
-    if (block_dim_x) *block_dim_x = bx;
-    if (block_dim_y) *block_dim_y = by;
-    if (width) *width = w;
-    if (height) *height = ht;
-    if (payload_bytes) *payload_bytes = expect;
-    return ASTC_OK;
+    return guarded([&]() -> int {
+        if (!path) return fail(ASTC_ERR_INVALID_ARG, "path is NULL");
+        FILE* f = std::fopen(path, "rb");
+        if (!f) return fail(ASTC_ERR_IO, std::string("cannot open: ") + path);
+        uint8_t h[kAstcHeaderBytes];
+        const size_t got = std::fread(h, 1, sizeof(h), f);
+        std::fseek(f, 0, SEEK_END);
+        const long size = std::ftell(f);
+        std::fclose(f);
+        if (got != sizeof(h)) return fail(ASTC_ERR_IO, "file shorter than the .astc header");
+        uint32_t magic = uint32_t(h[0]) | uint32_t(h[1]) << 8 | uint32_t(h[2]) << 16 | uint32_t(h[3]) << 24;
+        if (magic != kAstcMagic) return fail(ASTC_ERR_IO, "bad .astc magic");
+        const int bx = h[4], by = h[5];
+        const int w = h[7] | h[8] << 8 | h[9] << 16, ht = h[10] | h[11] << 8 | h[12] << 16;
+        if (bx < 4 || by < 4 || w < 1 || ht < 1) return fail(ASTC_ERR_IO, "bad .astc header fields");
+        const int64_t expect = 16LL * ((w + bx - 1) / bx) * ((ht + by - 1) / by);
+        if (size - (long)sizeof(h) != expect) return fail(ASTC_ERR_IO, "payload size does not match the header");
+        if (block_dim_x) *block_dim_x = bx;
+        if (block_dim_y) *block_dim_y = by;
+        if (width) *width = w;
+        if (height) *height = ht;
+        if (payload_bytes) *payload_bytes = expect;
+        return ASTC_OK;
+    });
 }
 
 int astc_mip_chain_info(const astc_image_desc* base, int max_levels, astc_image_desc* levels, int* count) {

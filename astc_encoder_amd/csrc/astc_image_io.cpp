@@ -162,7 +162,18 @@ int load_png(const char* path, std::vector<uint8_t>* rgba, int* wo, int* ho) {
     if (p.ctype == 3 && p.plte.empty()) return fail(ASTC_ERR_UNSUPPORTED, "palette PNG without PLTE");
     const int bpp_bits = channels_of(p.ctype) * p.depth;
 
-    // inflate
+    // filtered scanline bytes the header implies (every Adam7 pass when interlaced)
+    static const int ax0[7] = {0, 4, 0, 2, 0, 1, 0}, ay0[7] = {0, 0, 4, 0, 2, 0, 1};
+    static const int adx[7] = {8, 8, 4, 4, 2, 2, 1}, ady[7] = {8, 8, 8, 4, 4, 2, 2};
+    uint64_t expected = 0;
+    for (int k = 0; k < (p.interlace ? 7 : 1); ++k) {
+        const uint64_t dx = p.interlace ? adx[k] : 1, dy = p.interlace ? ady[k] : 1;
+        const uint64_t ox = p.interlace ? ax0[k] : 0, oy = p.interlace ? ay0[k] : 0;
+        const uint64_t pw = p.w > ox ? (p.w - ox + dx - 1) / dx : 0, ph = p.h > oy ? (p.h - oy + dy - 1) / dy : 0;
+        if (pw && ph) expected += ph * (1 + (pw * uint64_t(bpp_bits) + 7) / 8);
+    }
+
+    // inflate (never past what the header implies)
     std::vector<uint8_t> raw;
     z_stream zs;
     std::memset(&zs, 0, sizeof(zs));
@@ -179,6 +190,10 @@ int load_png(const char* path, std::vector<uint8_t>* rgba, int* wo, int* ho) {
             return fail(ASTC_ERR_IO, "corrupt PNG image data");
         }
         raw.insert(raw.end(), buf, buf + (sizeof(buf) - zs.avail_out));
+        if (raw.size() > expected) {
+            inflateEnd(&zs);
+            return fail(ASTC_ERR_IO, "PNG image data larger than its header allows");
+        }
     } while (zr != Z_STREAM_END && zs.avail_in > 0);
     inflateEnd(&zs);
 
@@ -216,21 +231,23 @@ int load_png(const char* path, std::vector<uint8_t>* rgba, int* wo, int* ho) {
 extern "C" {
 
 int astc_load_image(const char* path, int flip_vertically, int* width, int* height, uint8_t** rgba) {
-    if (!path || !width || !height || !rgba) return fail(ASTC_ERR_INVALID_ARG, "NULL argument");
-    *rgba = nullptr;
-    std::vector<uint8_t> img;
-    int w = 0, h = 0;
-    int rc = load_png(path, &img, &w, &h);
-    if (rc != ASTC_OK) return rc;
-    uint8_t* out = static_cast<uint8_t*>(std::malloc(img.size()));
-    if (!out) return fail(ASTC_ERR_NO_MEMORY, "host allocation failed");
-    const size_t row = size_t(w) * 4;
-    for (int y = 0; y < h; ++y)
-        std::memcpy(out + size_t(y) * row, img.data() + size_t(flip_vertically ? h - 1 - y : y) * row, row);
-    *width = w;
-    *height = h;
-    *rgba = out;
-    return ASTC_OK;
+    return guarded([&]() -> int {
+        if (!path || !width || !height || !rgba) return fail(ASTC_ERR_INVALID_ARG, "NULL argument");
+        *rgba = nullptr;
+        std::vector<uint8_t> img;
+        int w = 0, h = 0;
+        int rc = load_png(path, &img, &w, &h);
+        if (rc != ASTC_OK) return rc;
+        uint8_t* out = static_cast<uint8_t*>(std::malloc(img.size()));
+        if (!out) return fail(ASTC_ERR_NO_MEMORY, "host allocation failed");
+        const size_t row = size_t(w) * 4;
+        for (int y = 0; y < h; ++y)
+            std::memcpy(out + size_t(y) * row, img.data() + size_t(flip_vertically ? h - 1 - y : y) * row, row);
+        *width = w;
+        *height = h;
+        *rgba = out;
+        return ASTC_OK;
+    });
 }
 
 void astc_free_image(uint8_t* rgba) { std::free(rgba); }

@@ -5,6 +5,8 @@
 #include <stdint.h>
 
 #include <mutex>
+#include <new>
+#include <stdexcept>
 #include <string>
 
 #include "../../include/astc_mi355x.h"
@@ -33,6 +35,21 @@ struct ImageGeom {
     int bpp;
 };
 int check_desc(const astc_image_desc* d, ImageGeom* g);
+
+// Runs an entry point's body so that no C++ exception crosses the C ABI: a
+// failed host allocation becomes ASTC_ERR_NO_MEMORY.
+template <typename F>
+auto guarded(F&& f) -> decltype(f()) {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        return fail(ASTC_ERR_NO_MEMORY, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(ASTC_ERR_INVALID_ARG, std::string("unexpected error: ") + e.what());
+    } catch (...) {
+        return fail(ASTC_ERR_INVALID_ARG, "unexpected error");
+    }
+}
 
 // Host construction of the tables (identical on every host: sRGB via double pow,
 // rounded to binary32, times 255.0f; trits from the specification decoder).

@@ -213,77 +213,81 @@ int geometry(const astc_request& r, ReqGeom* g) {
 extern "C" {
 
 int64_t astc_request_bytes(const astc_image_desc* images, int nimages) {
-    astc_request r{};
-    r.images = images;
-    r.nimages = nimages;
-    r.src_dev = reinterpret_cast<const void*>(1);
-    ReqGeom g;
-    int rc = geometry(r, &g);
-    if (rc != ASTC_OK) return rc;
-    return g.src_bytes + g.out_bytes;
+    return guarded([&]() -> int64_t {
+        astc_request r{};
+        r.images = images;
+        r.nimages = nimages;
+        r.src_dev = reinterpret_cast<const void*>(1);
+        ReqGeom g;
+        int rc = geometry(r, &g);
+        if (rc != ASTC_OK) return rc;
+        return g.src_bytes + g.out_bytes;
+    });
 }
 
 int astc_server_create(astc_ctx* ctx, const astc_server_config* cfg, astc_server** out) {
-    if (!out) return fail(ASTC_ERR_INVALID_ARG, "out is NULL");
-    *out = nullptr;
-    if (!ctx || !cfg) return fail(ASTC_ERR_INVALID_ARG, "NULL argument");
-    const astc_tuning& t = cfg->tuning;
-    if (cfg->hbm_budget_bytes < kAlign || cfg->max_inflight < 1 || cfg->max_inflight > 8 ||
-        cfg->max_batch_requests < 1 ||
-        !(t.blocks_per_lane == 1 || t.blocks_per_lane == 2 || t.blocks_per_lane == 4 || t.blocks_per_lane == 8) ||
-        t.workgroups_per_cu < 1 || t.workgroups_per_cu > 4 || (t.prefetch != 0 && t.prefetch != 1) ||
-        (t.split_modes != 0 && t.split_modes != 1))
-        return fail(ASTC_ERR_INVALID_ARG, "server configuration out of range");
-    hipError_t e = hipSetDevice(ctx->device);
-    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-    astc_server* sv = new (std::nothrow) astc_server();
-    if (!sv) return fail(ASTC_ERR_NO_MEMORY, "host allocation failed");
-    sv->ctx = ctx;
-    sv->cfg = *cfg;
-    int rc = ASTC_OK;
-    if ((e = hipDeviceGetAttribute(&sv->num_cus, hipDeviceAttributeMultiprocessorCount, ctx->device)) != hipSuccess)
-        rc = hip_fail(e, "hipDeviceGetAttribute");
-    else if ((e = hipMalloc(&sv->ring_base, cfg->hbm_budget_bytes)) != hipSuccess)
-        rc = fail(ASTC_ERR_NO_MEMORY, std::string("ring allocation failed: ") + hipGetErrorString(e));
-    if (rc == ASTC_OK) {
-        sv->slots.resize(cfg->max_inflight);
-        for (Slot& s : sv->slots) {
-            if ((e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking)) != hipSuccess ||
-                (e = hipStreamCreateWithFlags(&s.side, hipStreamNonBlocking)) != hipSuccess ||
-                (e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming)) != hipSuccess ||
-                (e = hipEventCreateWithFlags(&s.fork, hipEventDisableTiming)) != hipSuccess ||
-                (e = hipEventCreateWithFlags(&s.join, hipEventDisableTiming)) != hipSuccess ||
-                (e = hipMalloc(&s.d_scalars, 4 * sizeof(int))) != hipSuccess ||
-                (e = hipHostMalloc(&s.h_scalars, 4 * sizeof(int), 0)) != hipSuccess) {
-                rc = hip_fail(e, "slot setup");
-                break;
+    return guarded([&]() -> int {
+        if (!out) return fail(ASTC_ERR_INVALID_ARG, "out is NULL");
+        *out = nullptr;
+        if (!ctx || !cfg) return fail(ASTC_ERR_INVALID_ARG, "NULL argument");
+        const astc_tuning& t = cfg->tuning;
+        if (cfg->hbm_budget_bytes < kAlign || cfg->max_inflight < 1 || cfg->max_inflight > 8 ||
+            cfg->max_batch_requests < 1 ||
+            !(t.blocks_per_lane == 1 || t.blocks_per_lane == 2 || t.blocks_per_lane == 4 || t.blocks_per_lane == 8) ||
+            t.workgroups_per_cu < 1 || t.workgroups_per_cu > 4 || (t.prefetch != 0 && t.prefetch != 1) ||
+            (t.split_modes != 0 && t.split_modes != 1))
+            return fail(ASTC_ERR_INVALID_ARG, "server configuration out of range");
+        hipError_t e = hipSetDevice(ctx->device);
+        if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+        astc_server* sv = new (std::nothrow) astc_server();
+        if (!sv) return fail(ASTC_ERR_NO_MEMORY, "host allocation failed");
+        sv->ctx = ctx;
+        sv->cfg = *cfg;
+        int rc = ASTC_OK;
+        if ((e = hipDeviceGetAttribute(&sv->num_cus, hipDeviceAttributeMultiprocessorCount, ctx->device)) != hipSuccess)
+            rc = hip_fail(e, "hipDeviceGetAttribute");
+        else if ((e = hipMalloc(&sv->ring_base, cfg->hbm_budget_bytes)) != hipSuccess)
+            rc = fail(ASTC_ERR_NO_MEMORY, std::string("ring allocation failed: ") + hipGetErrorString(e));
+        if (rc == ASTC_OK) {
+            sv->slots.resize(cfg->max_inflight);
+            for (Slot& s : sv->slots) {
+                if ((e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking)) != hipSuccess ||
+                    (e = hipStreamCreateWithFlags(&s.side, hipStreamNonBlocking)) != hipSuccess ||
+                    (e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming)) != hipSuccess ||
+                    (e = hipEventCreateWithFlags(&s.fork, hipEventDisableTiming)) != hipSuccess ||
+                    (e = hipEventCreateWithFlags(&s.join, hipEventDisableTiming)) != hipSuccess ||
+                    (e = hipMalloc(&s.d_scalars, 4 * sizeof(int))) != hipSuccess ||
+                    (e = hipHostMalloc(&s.h_scalars, 4 * sizeof(int), 0)) != hipSuccess) {
+                    rc = hip_fail(e, "slot setup");
+                    break;
+                }
+                // work-list room for a batch of ~1 G texels (2 MB per slot), so that
+                // regrowing (reallocation + graph capture, milliseconds) stays rare
+                if ((rc = ensure_capacity(sv, s, 1024, 65536, 65536)) != ASTC_OK) break;
             }
-            // work-list room for a batch of ~1 G texels (2 MB per slot), so that
-            // regrowing (reallocation + graph capture, milliseconds) stays rare
-            if ((rc = ensure_capacity(sv, s, 1024, 65536, 65536)) != ASTC_OK) break;
         }
-    }
-    // Replay every slot's graph once on an empty work list (the kernels exit at
-    // once), with a ring-to-ring copy and an event on the slot's stream, so that
-    // the runtime's first-use work (queue assignment, code-object and copy-kernel
-    // loading) happens here rather than under the first requests.
-    for (Slot& s : sv->slots) {
-        if (rc != ASTC_OK) break;
-        s.h_scalars[0] = s.h_scalars[1] = s.h_scalars[2] = s.h_scalars[3] = 0;
-        if ((e = hipMemcpyAsync(s.d_scalars, s.h_scalars, 4 * sizeof(int), hipMemcpyHostToDevice, s.stream)) !=
-                hipSuccess ||
-            (e = hipGraphLaunch(s.exec, s.stream)) != hipSuccess ||
-            (e = hipMemcpyAsync(sv->ring_base + 128, sv->ring_base, 128, hipMemcpyDeviceToDevice, s.stream)) !=
-                hipSuccess ||
-            (e = hipEventRecord(s.done, s.stream)) != hipSuccess || (e = hipEventSynchronize(s.done)) != hipSuccess)
-            rc = hip_fail(e, "slot warm-up");
-    }
-    if (rc != ASTC_OK) {
-        astc_server_destroy(sv);
-        return rc;
-    }
-    *out = sv;
-    return ASTC_OK;
+        // Replay every slot's graph once on an empty work list (the kernels exit at
+        // once), with a ring-to-ring copy and an event on the slot's stream, so that
+        // the runtime's first-use work (queue assignment, code-object and copy-kernel
+        // loading) happens here rather than under the first requests.
+        for (Slot& s : sv->slots) {
+            if (rc != ASTC_OK) break;
+            s.h_scalars[0] = s.h_scalars[1] = s.h_scalars[2] = s.h_scalars[3] = 0;
+            if ((e = hipMemcpyAsync(s.d_scalars, s.h_scalars, 4 * sizeof(int), hipMemcpyHostToDevice, s.stream)) !=
+                    hipSuccess ||
+                (e = hipGraphLaunch(s.exec, s.stream)) != hipSuccess ||
+                (e = hipMemcpyAsync(sv->ring_base + 128, sv->ring_base, 128, hipMemcpyDeviceToDevice, s.stream)) !=
+                    hipSuccess ||
+                (e = hipEventRecord(s.done, s.stream)) != hipSuccess || (e = hipEventSynchronize(s.done)) != hipSuccess)
+                rc = hip_fail(e, "slot warm-up");
+        }
+        if (rc != ASTC_OK) {
+            astc_server_destroy(sv);
+            return rc;
+        }
+        *out = sv;
+        return ASTC_OK;
+    });
 }
 
 int astc_server_destroy(astc_server* sv) {
@@ -308,160 +312,162 @@ int astc_server_destroy(astc_server* sv) {
 
 int astc_server_run_trace(astc_server* sv, const astc_request* reqs, int n, double* latency_s,
                           astc_server_stats* stats) {
-    if (!sv || (!reqs && n > 0) || !latency_s || n < 0) return fail(ASTC_ERR_INVALID_ARG, "NULL argument");
-    hipError_t e = hipSetDevice(sv->ctx->device);
-    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-    std::vector<ReqGeom> geo(n);
-    astc_server_stats st{};
-    for (int i = 0; i < n; ++i) {
-        int rc = geometry(reqs[i], &geo[i]);
-        if (rc != ASTC_OK) return fail(rc, "request " + std::to_string(i) + ": " + astc_last_error());
-        if (i && reqs[i].arrival_s < reqs[i - 1].arrival_s)
-            return fail(ASTC_ERR_INVALID_ARG, "arrival times must be non-decreasing");
-        if (geo[i].src_bytes + geo[i].out_bytes > sv->cfg.hbm_budget_bytes)
-            return fail(ASTC_ERR_BUDGET, "request " + std::to_string(i) + " is larger than the HBM budget");
-        st.images += reqs[i].nimages;
-        st.texels += geo[i].texels;
-        st.blocks += geo[i].blocks;
-    }
-    st.requests = n;
-    Ring ring(sv->cfg.hbm_budget_bytes);
-    std::deque<int> waiting;
-    int next = 0, done = 0, inflight = 0;
-    int64_t batched_reqs = 0;
-    std::vector<BatchLevel> levels;
-    std::vector<int> modes;
-    std::vector<BatchSeg> segs;
-    std::vector<BatchItem> items;
-    using clk = std::chrono::steady_clock;
-    const auto t0 = clk::now();
-    auto now = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
-    double last_done = 0.0, last_loop = 0.0;
-    int rc = ASTC_OK;
+    return guarded([&]() -> int {
+        if (!sv || (!reqs && n > 0) || !latency_s || n < 0) return fail(ASTC_ERR_INVALID_ARG, "NULL argument");
+        hipError_t e = hipSetDevice(sv->ctx->device);
+        if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+        std::vector<ReqGeom> geo(n);
+        astc_server_stats st{};
+        for (int i = 0; i < n; ++i) {
+            int rc = geometry(reqs[i], &geo[i]);
+            if (rc != ASTC_OK) return fail(rc, "request " + std::to_string(i) + ": " + astc_last_error());
+            if (i && reqs[i].arrival_s < reqs[i - 1].arrival_s)
+                return fail(ASTC_ERR_INVALID_ARG, "arrival times must be non-decreasing");
+            if (geo[i].src_bytes + geo[i].out_bytes > sv->cfg.hbm_budget_bytes)
+                return fail(ASTC_ERR_BUDGET, "request " + std::to_string(i) + " is larger than the HBM budget");
+            st.images += reqs[i].nimages;
+            st.texels += geo[i].texels;
+            st.blocks += geo[i].blocks;
+        }
+        st.requests = n;
+        Ring ring(sv->cfg.hbm_budget_bytes);
+        std::deque<int> waiting;
+        int next = 0, done = 0, inflight = 0;
+        int64_t batched_reqs = 0;
+        std::vector<BatchLevel> levels;
+        std::vector<int> modes;
+        std::vector<BatchSeg> segs;
+        std::vector<BatchItem> items;
+        using clk = std::chrono::steady_clock;
+        const auto t0 = clk::now();
+        auto now = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
+        double last_done = 0.0, last_loop = 0.0;
+        int rc = ASTC_OK;
 
-    while (done < n && rc == ASTC_OK) {
-        bool progressed = false;
-        // completions
-        for (Slot& s : sv->slots) {
-            if (!s.busy) continue;
-            e = hipEventQuery(s.done);
-            if (e == hipErrorNotReady) continue;
-            if (e != hipSuccess) {
-                rc = hip_fail(e, "batch event");
-                break;
-            }
-            const double t = now();
-            for (size_t k = 0; k < s.reqs.size(); ++k) {
-                latency_s[s.reqs[k]] = t - reqs[s.reqs[k]].arrival_s;
-                ring.release(s.offs[k]);
-            }
-            done += (int)s.reqs.size();
-            last_done = t;
-            s.busy = false;
-            --inflight;
-            progressed = true;
-        }
-        if (rc != ASTC_OK) break;
-        // arrivals
-        const double t = now();
-        while (next < n && reqs[next].arrival_s <= t) waiting.push_back(next++);
-        // launches
-        for (Slot& s : sv->slots) {
-            if (s.busy || waiting.empty()) continue;
-            s.reqs.clear();
-            s.offs.clear();
-            levels.clear();
-            modes.clear();
-            while (!waiting.empty() && (int)s.reqs.size() < sv->cfg.max_batch_requests) {
-                const int r = waiting.front();
-                int64_t off;
-                if (!ring.alloc(geo[r].src_bytes + geo[r].out_bytes, &off)) break;
-                waiting.pop_front();
-                s.reqs.push_back(r);
-                s.offs.push_back(off);
-                uint8_t* base = sv->ring_base + off;
-                for (size_t i = 0; i < geo[r].levels.size(); ++i) {
-                    BatchLevel L = geo[r].levels[i];
-                    L.f.src = base + reinterpret_cast<uintptr_t>(L.f.src);
-                    L.dst = reinterpret_cast<uint32_t*>(base + geo[r].src_bytes + reinterpret_cast<uintptr_t>(L.dst));
-                    levels.push_back(L);
-                    modes.push_back(geo[r].modes[i]);
+        while (done < n && rc == ASTC_OK) {
+            bool progressed = false;
+            // completions
+            for (Slot& s : sv->slots) {
+                if (!s.busy) continue;
+                e = hipEventQuery(s.done);
+                if (e == hipErrorNotReady) continue;
+                if (e != hipSuccess) {
+                    rc = hip_fail(e, "batch event");
+                    break;
                 }
+                const double t = now();
+                for (size_t k = 0; k < s.reqs.size(); ++k) {
+                    latency_s[s.reqs[k]] = t - reqs[s.reqs[k]].arrival_s;
+                    ring.release(s.offs[k]);
+                }
+                done += (int)s.reqs.size();
+                last_done = t;
+                s.busy = false;
+                --inflight;
+                progressed = true;
             }
-            if (s.reqs.empty()) break;  // the ring is full: wait for completions
-            int first4 = 0;
-            plan_work(levels, modes, sv->cfg.tuning.blocks_per_lane, &segs, &items, &first4);
-            if (levels.size() > s.cap_levels || segs.size() > s.cap_segs || items.size() > s.cap_items) {
-                // grow every idle slot at once: regrowing (allocation + graph capture)
-                // costs milliseconds, so it should not recur slot by slot mid-stream
-                for (Slot& o : sv->slots)
-                    if (!o.busy && rc == ASTC_OK)
-                        rc = ensure_capacity(sv, o, std::max(levels.size(), o.cap_levels),
-                                             std::max(segs.size(), o.cap_segs), std::max(items.size(), o.cap_items));
-                if (rc != ASTC_OK) break;
+            if (rc != ASTC_OK) break;
+            // arrivals
+            const double t = now();
+            while (next < n && reqs[next].arrival_s <= t) waiting.push_back(next++);
+            // launches
+            for (Slot& s : sv->slots) {
+                if (s.busy || waiting.empty()) continue;
+                s.reqs.clear();
+                s.offs.clear();
+                levels.clear();
+                modes.clear();
+                while (!waiting.empty() && (int)s.reqs.size() < sv->cfg.max_batch_requests) {
+                    const int r = waiting.front();
+                    int64_t off;
+                    if (!ring.alloc(geo[r].src_bytes + geo[r].out_bytes, &off)) break;
+                    waiting.pop_front();
+                    s.reqs.push_back(r);
+                    s.offs.push_back(off);
+                    uint8_t* base = sv->ring_base + off;
+                    for (size_t i = 0; i < geo[r].levels.size(); ++i) {
+                        BatchLevel L = geo[r].levels[i];
+                        L.f.src = base + reinterpret_cast<uintptr_t>(L.f.src);
+                        L.dst = reinterpret_cast<uint32_t*>(base + geo[r].src_bytes + reinterpret_cast<uintptr_t>(L.dst));
+                        levels.push_back(L);
+                        modes.push_back(geo[r].modes[i]);
+                    }
+                }
+                if (s.reqs.empty()) break;  // the ring is full: wait for completions
+                int first4 = 0;
+                plan_work(levels, modes, sv->cfg.tuning.blocks_per_lane, &segs, &items, &first4);
+                if (levels.size() > s.cap_levels || segs.size() > s.cap_segs || items.size() > s.cap_items) {
+                    // grow every idle slot at once: regrowing (allocation + graph capture)
+                    // costs milliseconds, so it should not recur slot by slot mid-stream
+                    for (Slot& o : sv->slots)
+                        if (!o.busy && rc == ASTC_OK)
+                            rc = ensure_capacity(sv, o, std::max(levels.size(), o.cap_levels),
+                                                 std::max(segs.size(), o.cap_segs), std::max(items.size(), o.cap_items));
+                    if (rc != ASTC_OK) break;
+                }
+                std::memcpy(s.h_levels, levels.data(), levels.size() * sizeof(BatchLevel));
+                std::memcpy(s.h_segs, segs.data(), segs.size() * sizeof(BatchSeg));
+                std::memcpy(s.h_items, items.data(), items.size() * sizeof(BatchItem));
+                s.h_scalars[0] = (int)items.size();
+                s.h_scalars[1] = first4;
+                s.h_scalars[2] = 0;
+                s.h_scalars[3] = 0;
+                e = hipSuccess;
+                for (size_t k = 0; k < s.reqs.size() && e == hipSuccess; ++k) {
+                    const int r = s.reqs[k];
+                    e = hipMemcpyAsync(sv->ring_base + s.offs[k], reqs[r].src_dev, geo[r].src_bytes,
+                                       hipMemcpyDeviceToDevice, s.stream);
+                }
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync(s.d_levels, s.h_levels, levels.size() * sizeof(BatchLevel), hipMemcpyHostToDevice,
+                                       s.stream);
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync(s.d_segs, s.h_segs, segs.size() * sizeof(BatchSeg), hipMemcpyHostToDevice,
+                                       s.stream);
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync(s.d_items, s.h_items, items.size() * sizeof(BatchItem), hipMemcpyHostToDevice,
+                                       s.stream);
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync(s.d_scalars, s.h_scalars, 4 * sizeof(int), hipMemcpyHostToDevice, s.stream);
+                if (e == hipSuccess) e = hipGraphLaunch(s.exec, s.stream);
+                for (size_t k = 0; k < s.reqs.size() && e == hipSuccess; ++k) {
+                    const int r = s.reqs[k];
+                    if (reqs[r].dst_dev)
+                        e = hipMemcpyAsync(reqs[r].dst_dev, sv->ring_base + s.offs[k] + geo[r].src_bytes,
+                                           geo[r].out_bytes, hipMemcpyDeviceToDevice, s.stream);
+                }
+                if (e == hipSuccess) e = hipEventRecord(s.done, s.stream);
+                if (e != hipSuccess) {
+                    rc = hip_fail(e, "batch submission");
+                    break;
+                }
+                s.busy = true;
+                ++inflight;
+                ++st.batches;
+                batched_reqs += (int64_t)s.reqs.size();
+                st.peak_inflight = std::max(st.peak_inflight, inflight);
+                progressed = true;
             }
-            std::memcpy(s.h_levels, levels.data(), levels.size() * sizeof(BatchLevel));
-            std::memcpy(s.h_segs, segs.data(), segs.size() * sizeof(BatchSeg));
-            std::memcpy(s.h_items, items.data(), items.size() * sizeof(BatchItem));
-            s.h_scalars[0] = (int)items.size();
-            s.h_scalars[1] = first4;
-            s.h_scalars[2] = 0;
-            s.h_scalars[3] = 0;
-            e = hipSuccess;
-            for (size_t k = 0; k < s.reqs.size() && e == hipSuccess; ++k) {
-                const int r = s.reqs[k];
-                e = hipMemcpyAsync(sv->ring_base + s.offs[k], reqs[r].src_dev, geo[r].src_bytes,
-                                   hipMemcpyDeviceToDevice, s.stream);
+            if (rc != ASTC_OK) break;
+            // Idle until the next arrival when nothing is on the GPU: a sleep can
+            // overshoot by milliseconds, so only long gaps sleep (waking 2 ms early)
+            // and the rest is polled.
+            const double tl = now();
+            st.max_loop_gap_s = std::max(st.max_loop_gap_s, tl - last_loop);
+            if (!progressed && inflight == 0 && waiting.empty() && next < n) {
+                const double gap = reqs[next].arrival_s - tl;
+                if (gap > 3e-3) std::this_thread::sleep_for(std::chrono::duration<double>(gap - 2e-3));
             }
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(s.d_levels, s.h_levels, levels.size() * sizeof(BatchLevel), hipMemcpyHostToDevice,
-                                   s.stream);
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(s.d_segs, s.h_segs, segs.size() * sizeof(BatchSeg), hipMemcpyHostToDevice,
-                                   s.stream);
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(s.d_items, s.h_items, items.size() * sizeof(BatchItem), hipMemcpyHostToDevice,
-                                   s.stream);
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(s.d_scalars, s.h_scalars, 4 * sizeof(int), hipMemcpyHostToDevice, s.stream);
-            if (e == hipSuccess) e = hipGraphLaunch(s.exec, s.stream);
-            for (size_t k = 0; k < s.reqs.size() && e == hipSuccess; ++k) {
-                const int r = s.reqs[k];
-                if (reqs[r].dst_dev)
-                    e = hipMemcpyAsync(reqs[r].dst_dev, sv->ring_base + s.offs[k] + geo[r].src_bytes,
-                                       geo[r].out_bytes, hipMemcpyDeviceToDevice, s.stream);
-            }
-            if (e == hipSuccess) e = hipEventRecord(s.done, s.stream);
-            if (e != hipSuccess) {
-                rc = hip_fail(e, "batch submission");
-                break;
-            }
-            s.busy = true;
-            ++inflight;
-            ++st.batches;
-            batched_reqs += (int64_t)s.reqs.size();
-            st.peak_inflight = std::max(st.peak_inflight, inflight);
-            progressed = true;
+            last_loop = now();
         }
-        if (rc != ASTC_OK) break;
-        // Idle until the next arrival when nothing is on the GPU: a sleep can
-        // overshoot by milliseconds, so only long gaps sleep (waking 2 ms early)
-        // and the rest is polled.
-        const double tl = now();
-        st.max_loop_gap_s = std::max(st.max_loop_gap_s, tl - last_loop);
-        if (!progressed && inflight == 0 && waiting.empty() && next < n) {
-            const double gap = reqs[next].arrival_s - tl;
-            if (gap > 3e-3) std::this_thread::sleep_for(std::chrono::duration<double>(gap - 2e-3));
-        }
-        last_loop = now();
-    }
-    for (Slot& s : sv->slots) (void)hipStreamSynchronize(s.stream);
-    if (rc != ASTC_OK) return rc;
-    st.wall_s = n ? last_done - reqs[0].arrival_s : 0.0;
-    st.mean_batch_requests = st.batches ? double(batched_reqs) / st.batches : 0.0;
-    st.peak_ring_bytes = ring.peak();
-    if (stats) *stats = st;
-    return ASTC_OK;
+        for (Slot& s : sv->slots) (void)hipStreamSynchronize(s.stream);
+        if (rc != ASTC_OK) return rc;
+        st.wall_s = n ? last_done - reqs[0].arrival_s : 0.0;
+        st.mean_batch_requests = st.batches ? double(batched_reqs) / st.batches : 0.0;
+        st.peak_ring_bytes = ring.peak();
+        if (stats) *stats = st;
+        return ASTC_OK;
+    });
 }
 
 }  // extern "C"

@@ -107,6 +107,19 @@ def test_load_errors(tmp_path):
         astc.load_tex(bad)
 
 
+def test_image_data_beyond_the_header_is_refused(tmp_path):
+    """A 2x2 RGBA header followed by 1 MiB of compressed zeros (a decompression
+    bomb in miniature) is rejected instead of inflated."""
+    def chunk(tag, data):
+        return struct.pack(">I", len(data)) + tag + data + struct.pack(">I", zlib.crc32(tag + data) & 0xFFFFFFFF)
+    png = b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", 2, 2, 8, 6, 0, 0, 0))
+    png += chunk(b"IDAT", zlib.compress(bytes(1 << 20), 9)) + chunk(b"IEND", b"")
+    p = tmp_path / "bomb.png"
+    p.write_bytes(png)
+    with pytest.raises(astc.AstcError, match="larger than its header"):
+        astc.load_tex(p)
+
+
 def test_cli_without_gpu_paths(tmp_path):
     """Argument count and load failures behave as main.cpp:182-185, 215-218."""
     import subprocess
