@@ -243,6 +243,21 @@ def cpu_baseline(wl, img, gpu_blocks, budget_s=20.0):
                 gpu_block_match=match, oracle_psnr_db=round(orc.psnr(img, wl["flags"], dec), 4))
 
 
+def cpu_small_case(wl):
+    """The oracle on the 1024x1024 RGB8 4x4 texture (all host cores, median of 5)."""
+    from tests import oracle_ref as orc
+    cores = len(os.sched_getaffinity(0))
+    img = wl["gen"](0, wl["w"], wl["h"])
+    orc.oracle_encode(img, wl["dim"], wl["flags"], threads=cores)
+    times = []
+    for _ in range(5):
+        t = time.perf_counter()
+        orc.oracle_encode(img, wl["dim"], wl["flags"], threads=cores)
+        times.append(time.perf_counter() - t)
+    return {"workload": wl["name"], "mtexels_s": round(wl["w"] * wl["h"] / float(np.median(times)) / 1e6, 1),
+            "cores": cores}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -307,6 +322,8 @@ def main():
     line["psnr_db"] = r["psnr_db"]
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(head, r["img"], r["blocks"])
+        small = wls["sinusoid_rgb8_4x4"]  # the reference's CPU-sized case: 1024^2 RGB8 4x4
+        line["cpu_baseline"]["small_case"] = cpu_small_case(small)
         line["psnr_delta_vs_oracle_db"] = round(r["psnr_db"] - line["cpu_baseline"].pop("oracle_psnr_db"), 6)
     if not args.no_extras:
         extras = {}
