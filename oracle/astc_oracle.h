@@ -1,7 +1,7 @@
 /*
  * astc_oracle.h -- CPU oracle for the ASTC block-encode path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP encoder
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle that the HIP encoder
  * is compared against; it is never linked into, loaded by, or called from the
  * product path (astc_encoder_amd/).  Only tests/, __graft_entry__.smoke() and
  * the cpu_baseline leg of bench.py may load it.

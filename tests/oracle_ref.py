@@ -1,6 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/_build/liboracle.so) for tests.
 
-Test infrastructure: the oracle is the checker, never part of the product path.
+Test infrastructure: the oracle is what the HIP path is compared against, never part
+of the product path.
 """
 import ctypes
 import os
