@@ -77,6 +77,22 @@ def test_batch_matches_plain_and_oracle(enc):
     b.close()
 
 
+def test_batch_with_largest_dimensions(enc):
+    """A row and a column of 2^24 - 1 texels (the .astc header's limit) beside
+    small images: thousands of work items from one image, every block padded."""
+    rng = np.random.default_rng(24)
+    wide = torch.from_numpy(rng.integers(0, 256, (1, 16777215, 4), dtype=np.uint8)).cuda()
+    tall = torch.from_numpy(rng.integers(0, 256, (16777215, 1, 3), dtype=np.uint8)).cuda()
+    images = [(wide, 4, astc.FLAG_ALPHA), (tall, 6, astc.FLAG_SRGB)] + mixed_images(enc, 6, 4)
+    want = plain_outputs(enc, images)
+    b = astc.Batch(enc, images)
+    b.launch()
+    torch.cuda.synchronize()
+    for i, (o, w) in enumerate(zip(b.outputs, want)):
+        np.testing.assert_array_equal(o.cpu().numpy(), w, err_msg=f"image {i}")
+    b.close()
+
+
 @pytest.mark.parametrize("bpl,wpc,pf,split", [(1, 1, 0, 0), (2, 2, 1, 0), (4, 2, 0, 0), (8, 1, 1, 0), (8, 4, 1, 0),
                                              (1, 1, 0, 1), (2, 2, 1, 1), (4, 2, 1, 1), (8, 4, 0, 1)])
 def test_every_tuning_gives_the_same_blocks(enc, bpl, wpc, pf, split):

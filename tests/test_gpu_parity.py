@@ -104,6 +104,17 @@ def test_max_accumulation_search(enc, dim, channels, flags, shape):
     assert_same(gpu_encode(enc, img, dim, f), orc.oracle_encode(img, dim, f), f"{dim} {channels} {f} {shape}")
 
 
+@pytest.mark.parametrize("shape,dim,flags", [((1, 16777215), 4, astc.FLAG_ALPHA),
+                                              ((16777215, 1), 6, astc.FLAG_ALPHA | astc.FLAG_SRGB)])
+def test_largest_dimensions(enc, shape, dim, flags):
+    """2^24 - 1 texels (the .astc header's 24-bit limit) in one row or in one
+    column: 4.2 M and 2.8 M blocks, every one of them padded."""
+    h, w = shape
+    img = np.random.default_rng(w + h + dim).integers(0, 256, size=(h, w, 4), dtype=np.uint8)
+    print(f"case {w}x{h} dim={dim} flags={flags}", flush=True)
+    assert_same(gpu_encode(enc, img, dim, flags), orc.oracle_encode(img, dim, flags), f"{w}x{h}")
+
+
 @pytest.mark.parametrize("channels,dim", [(4, 4), (3, 6), (2, 6), (3, 4)])
 def test_unaligned_source_and_padded_pitch(enc, channels, dim):
     """Source pointer off by one byte and an odd row pitch take the guarded path."""
