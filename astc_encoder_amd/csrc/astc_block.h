@@ -200,9 +200,11 @@ struct PackTables { uint32_t spread[2][256]; };
 
 // wsh[i]: twice weight i's quantised value in the low five bits (the field
 // offset into the split tables).
+// swap: the endpoints go in the other order (e1 first), which exchanges the two
+// bytes of every endpoint pair.
 template <bool ALPHA>
 __device__ __forceinline__ Block128 pack_block(const uint32_t e0q[4], const uint32_t e1q[4], const uint32_t wsh[16],
-                                               const PackTables* pt) {
+                                               const PackTables* pt, bool swap) {
     constexpr WeightSplit split = ALPHA ? kSplitQ6 : kSplitQ12;
     constexpr int b = split.bits;          // bits per weight below the trit
     constexpr int group_bits = 5 * b + 8;  // one 5-value trit group
@@ -228,8 +230,12 @@ __device__ __forceinline__ Block128 pack_block(const uint32_t e0q[4], const uint
         stream |= grp << (g * group_bits);
     }
     // endpoints: QUANT_256 raw bytes r0 r1 g0 g1 b0 b1 [a0 a1] from block bit 17
-    const uint32_t ep_x = e0q[0] | e1q[0] << 8 | e0q[1] << 16 | e1q[1] << 24;
-    const uint32_t ep_y = ALPHA ? (e0q[2] | e1q[2] << 8 | e0q[3] << 16 | e1q[3] << 24) : (e0q[2] | e1q[2] << 8);
+    uint32_t ep_x = e0q[0] | e1q[0] << 8 | e0q[1] << 16 | e1q[1] << 24;
+    uint32_t ep_y = ALPHA ? (e0q[2] | e1q[2] << 8 | e0q[3] << 16 | e1q[3] << 24) : (e0q[2] | e1q[2] << 8);
+    if (swap) {
+        ep_x = (ep_x & 0x00FF00FFu) << 8 | (ep_x >> 8 & 0x00FF00FFu);
+        ep_y = (ep_y & 0x00FF00FFu) << 8 | (ep_y >> 8 & 0x00FF00FFu);
+    }
     constexpr uint32_t mode = ALPHA ? kModeRGBA : kModeRGB;
     constexpr uint32_t cem = ALPHA ? CEM_LDR_RGBA_DIRECT : CEM_LDR_RGB_DIRECT;
     Block128 blk;
@@ -404,16 +410,14 @@ __device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], co
         e0[c] = clamp255(fma_(v[c], lo, mean[c]));
         e1[c] = clamp255(fma_(v[c], hi, mean[c]));
     }
+    // The shader swaps e0 and e1 when e0's rounded RGB sum is the larger
+    // (:125-130).  Swapping only reverses each endpoint byte pair and negates
+    // e1 - e0; negation is exact and round-to-nearest is symmetric in sign, so
+    // every later quantity of the swapped block is the unswapped one with its
+    // sign flipped.  The swap is carried as that flag instead of moving values.
     const float s0 = (rintf(e0[0]) + rintf(e0[1])) + rintf(e0[2]);
     const float s1 = (rintf(e1[0]) + rintf(e1[1])) + rintf(e1[2]);
-    if (s0 > s1) {
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            const float t = e0[c];
-            e0[c] = e1[c];
-            e1[c] = t;
-        }
-    }
+    const bool swap = s0 > s1;
     if (!ALPHA) {
         e0[3] = 255.0f;
         e1[3] = 255.0f;
@@ -439,11 +443,13 @@ __device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], co
 #pragma unroll
     for (int c = 0; c < KC; ++c) k[c] = e1[c] - e0[c];
     const float kk = dotn<KC>(k, k);
-    if (is_short(kk)) {
-#pragma unroll
-        for (int i = 0; i < 16; ++i) wsh[i] = 0u;
-    } else {
-        const float s = rsqrt_nr(kk);
+    // |e1 - e0| < 1e-5 gives all-zero weights (:323-330).  A zero scale gives
+    // the same fields without a branch: every projection is +-0, the range is 0,
+    // so each (p - min) * 1e5 is +-0 and rounds to weight 0.  With the swap, the
+    // scale is negated (e1 - e0 of the swapped block).
+    {
+        float s = is_short(kk) ? 0.0f : rsqrt_nr(kk);
+        s = swap ? -s : s;
 #pragma unroll
         for (int c = 0; c < KC; ++c) k[c] = k[c] * s;
         float proj[16];
@@ -481,7 +487,7 @@ __device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], co
             wsh[i] = __float_as_uint(wn * float(range) + 12582912.0f) << 1;
         }
     }
-    return pack_block<ALPHA>(e0q, e1q, wsh, pack);
+    return pack_block<ALPHA>(e0q, e1q, wsh, pack, swap);
 }
 
 }  // namespace astc

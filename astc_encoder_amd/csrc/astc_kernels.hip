@@ -59,12 +59,17 @@ __host__ __device__ inline int channel_class(int fmt, uint32_t flags, int width,
 
 // SRGB: 1 / 0 fixes the fetch at compile time, -1 selects the table per image
 // (sRGB or identity, no branch).  The channel class CH is the number of active
-// channels the encoder works on.
-template <int DIM, int SRGB, int CH>
+// channels the encoder works on.  OVERRIDE applies the normal-map override
+// (blue = alpha = 255) to the texel words.  Only an encoder that reads blue or
+// alpha of a normal map needs it: the principal component encoders take every
+// normal map through the two-channel class, which reads red and green only, and
+// the mask is zero for every other image, so only the four-channel
+// max-accumulation search sets it.
+template <int DIM, int SRGB, int CH, bool OVERRIDE = false>
 __device__ __forceinline__ void unpack_texels(const uint32_t p[DIM * DIM], const FetchArgs& f, const float* lut,
                                               float x[DIM * DIM][4]) {
     const float* colour = SRGB < 0 ? lut + ((f.flags & FLAG_SRGB) ? 0 : 256) : SRGB == 1 ? lut : lut + 256;
-    unpack_block<DIM, CH>(p, f.or_mask, colour, lut + 256, x);
+    unpack_block<DIM, CH>(p, OVERRIDE ? f.or_mask : 0u, colour, lut + 256, x);
 }
 
 __device__ __forceinline__ void store_block(uint32_t* dst, int id, const Block128& b) {
@@ -86,7 +91,7 @@ __device__ __forceinline__ void encode_loaded(const FetchArgs& f, int xblocks, u
     for (int o = threadIdx.x;; o += stride) {
         const bool has_next = o + stride < count;
         float x[DIM * DIM][4];
-        unpack_texels<DIM, SRGB, CH>(cur, f, lut, x);
+        unpack_texels<DIM, SRGB, CH, METHOD == METHOD_MAX_ACCUMULATION>(cur, f, lut, x);
         // the packed words are dead once converted: the next block's loads reuse them
         if (has_next && prefetch) load_block_id<DIM>(f, xblocks, first + o + stride, cur);
         store_block(dst, first + o, encode_block<DIM, ALPHA, CH, METHOD>(x, trit));
