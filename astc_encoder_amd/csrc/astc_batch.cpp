@@ -192,8 +192,8 @@ int astc_batch_create(astc_ctx* ctx, const astc_image_desc* descs, const void* c
             L.f.flags = descs[i].flags;
             L.f.aligned = ((reinterpret_cast<uintptr_t>(srcs[i]) & 3u) == 0) && ((descs[i].row_pitch & 3) == 0);
             L.f.or_mask = normal_or_mask(descs[i].flags);
+            set_block_row(&L.f, g.xblocks);
             L.dst = static_cast<uint32_t*>(dsts[i]);
-            L.xblocks = g.xblocks;
             L.nblocks = (int)g.nblocks;
             b->modes[i] = mode_of(descs[i].block_dim, descs[i].src_format, descs[i].flags, descs[i].width, descs[i].height);
             b->blocks += g.nblocks;

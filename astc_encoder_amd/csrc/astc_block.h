@@ -90,8 +90,11 @@ struct FetchArgs {
     const uint8_t* src;
     int width, height, row_pitch, fmt;
     uint32_t flags;
-    int aligned;       // src and row_pitch are multiples of 4
-    uint32_t or_mask;  // 0xFFFF0000 for normal maps (blue = alpha = 255), else 0
+    int aligned;        // src and row_pitch are multiples of 4
+    uint32_t or_mask;   // 0xFFFF0000 for normal maps (blue = alpha = 255), else 0
+    int xblocks;        // blocks per block row
+    uint32_t xb_magic;  // block id / xblocks = mulhi(2 id, xb_magic) >> xb_shift
+    int xb_shift;
 };
 
 // OR-mask that applies the normal-map override to a packed RGBA8 texel.  Setting
@@ -100,7 +103,54 @@ struct FetchArgs {
 // 255.0f, and out-of-bounds (zero) texels get blue = alpha = 255 as well.
 __host__ __device__ inline uint32_t normal_or_mask(uint32_t flags) { return (flags & FLAG_NORMAL) ? 0xFFFF0000u : 0u; }
 
-__device__ __forceinline__ int bytes_per_texel(int fmt) { return fmt == FMT_RGBA8 ? 4 : fmt == FMT_RGB8 ? 3 : 2; }
+// Division of a block id n < 2^31 by d = xblocks without a divide: with
+// l = ceil(log2 d) and m = ceil(2^(31+l) / d) (which fits 32 bits),
+// floor(n m / 2^(31+l)) = floor(n / d) for every n < 2^31, because
+// m d - 2^(31+l) < d <= 2^l keeps the error term below 1/d (Granlund and
+// Montgomery, "Division by invariant integers using multiplication", 1994).
+// mulhi(2n, m) >> l is that quotient.
+__host__ inline void set_block_row(FetchArgs* f, int xblocks) {
+    int l = 0;
+    while ((uint64_t(1) << l) < uint64_t(xblocks)) ++l;
+    f->xblocks = xblocks;
+    f->xb_shift = l;
+    f->xb_magic = uint32_t(((uint64_t(1) << (31 + l)) + uint64_t(xblocks) - 1) / uint64_t(xblocks));
+}
+
+__device__ __forceinline__ int block_row_of(const FetchArgs& f, int id) {
+    return int(__umulhi(uint32_t(id) << 1, f.xb_magic) >> f.xb_shift);
+}
+
+// Channel classes: which channels are 255 in every texel of every block.
+//   CH_ALL    any content: four channels;
+//   CH_RGB8   an RGB8 image tiled exactly by its blocks (no texel outside the
+//             image, where alpha would be 0): alpha is constant, three channels;
+//   CH_NORMAL normal maps (blue = alpha = 255, outside the image too): two.
+enum : int { CH_ALL = 4, CH_RGB8 = 3, CH_NORMAL = 2 };
+
+// A block's texels as the encoder receives them from the fetch, by class:
+//   CH_ALL    one packed RGBA8 word per texel;
+//   CH_RGB8   each texel row's RGB bytes back to back (the source bytes);
+//   CH_NORMAL each texel row's red and green bytes back to back (for an RG8
+//             source, the source bytes).
+// Texel k of row r = k / DIM, column i = k % DIM, has channel c at byte
+// i * BPT + c of its row's WPR words.  The conversion picks the bytes out of the
+// words with byte selects, so RGB8 and RG8 rows need no realignment.
+template <int DIM, int CH>
+struct Layout {
+    static constexpr int BPT = CH == CH_RGB8 ? 3 : CH == CH_NORMAL ? 2 : 4;  // bytes per texel
+    static constexpr int WPR = (DIM * BPT + 3) / 4;                           // words per row
+    static constexpr int WORDS = DIM * WPR;
+};
+
+template <int DIM, int CH>
+__device__ __forceinline__ uint32_t layout_byte(const uint32_t* w, int k, int c) {
+    using L = Layout<DIM, CH>;
+    const int j = (k % DIM) * L::BPT + c;
+    return (w[(k / DIM) * L::WPR + (j >> 2)] >> (8 * (j & 3))) & 0xFFu;
+}
+
+__host__ __device__ inline int bytes_per_texel(int fmt) { return fmt == FMT_RGBA8 ? 4 : fmt == FMT_RGB8 ? 3 : 2; }
 
 // Packed RGBA8 of texel (x, y) with bounds check (slow path).
 __device__ __forceinline__ uint32_t load_texel_guarded(const FetchArgs& a, int x, int y) {
@@ -111,12 +161,13 @@ __device__ __forceinline__ uint32_t load_texel_guarded(const FetchArgs& a, int x
     return uint32_t(p[0]) | uint32_t(p[1]) << 8 | 0xFF000000u;
 }
 
-// One texel row of a fully in-bounds block from 4-byte-aligned words.
+// One texel row as packed RGBA8 words, from 4-byte-aligned words of a row that
+// starts at `row` (any source format).
 template <int DIM>
-__device__ __forceinline__ void load_row_fast(const FetchArgs& a, int x0, int y, uint32_t out[DIM]) {
+__device__ __forceinline__ void load_row_rgba(const FetchArgs& a, const uint8_t* row, uint32_t out[DIM]) {
     const int bpp = bytes_per_texel(a.fmt);
-    const size_t off = (size_t)y * a.row_pitch + (size_t)x0 * bpp;
-    const uint32_t* w = reinterpret_cast<const uint32_t*>(a.src + (off & ~size_t(3)));
+    const uint32_t shift = uint32_t(reinterpret_cast<uintptr_t>(row) & 3u);
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(row - shift);
     if (a.fmt == FMT_RGBA8) {
 #pragma unroll
         for (int i = 0; i < DIM; ++i) out[i] = w[i];
@@ -125,7 +176,6 @@ __device__ __forceinline__ void load_row_fast(const FetchArgs& a, int x0, int y,
     // RGB8 / RG8: the row's bytes start `shift` bytes into the first word; the
     // words are realigned with a funnel shift so that byte j sits at a fixed place.
     constexpr int kWords = (DIM * 3 + 3) / 4 + 1;
-    const uint32_t shift = uint32_t(off & 3);
     const int nwords = int((DIM * bpp + shift + 3) >> 2);
     uint32_t raw[kWords + 1];
 #pragma unroll
@@ -144,21 +194,46 @@ __device__ __forceinline__ void load_row_fast(const FetchArgs& a, int x0, int y,
     }
 }
 
-// RGBA8 -> floats (x255 scale) through LDS tables: `colour` converts the colour
-// channels (sRGB or UNORM), `unorm` the alpha.  Only the first NC channels are
-// converted (the others are known constants to the encoder, see encode_block).
-// The normal-map override is already in the word (FetchArgs::or_mask).
-template <int NC = 4>
-__device__ __forceinline__ void unpack_texel(uint32_t p, const float* colour, const float* unorm, float t[4]) {
+// One texel row of a fully in-bounds block in the class's layout.
+template <int DIM, int CH>
+__device__ __forceinline__ void load_row_fast(const FetchArgs& a, const uint8_t* row, uint32_t* out) {
+    using L = Layout<DIM, CH>;
+    if (CH == CH_ALL) {
+        load_row_rgba<DIM>(a, row, out);
+    } else if (CH == CH_RGB8 && DIM == 4) {
+        // 12 bytes at 12 bx past an aligned row start: three aligned words
+        const uint32_t* w = reinterpret_cast<const uint32_t*>(row);
 #pragma unroll
-    for (int c = 0; c < NC && c < 3; ++c) t[c] = colour[(p >> (8 * c)) & 0xFFu];
-    if (NC == 4) t[3] = unorm[p >> 24];
+        for (int i = 0; i < L::WPR; ++i) out[i] = w[i];
+    } else if (CH == CH_RGB8) {
+        // 18 bytes at 18 bx: they start 0 or 2 bytes into an aligned word
+        const uint32_t shift = uint32_t(reinterpret_cast<uintptr_t>(row) & 3u);
+        const uint32_t* w = reinterpret_cast<const uint32_t*>(row - shift);
+        uint32_t raw[L::WPR + 1];
+#pragma unroll
+        for (int i = 0; i < L::WPR; ++i) raw[i] = w[i];
+        raw[L::WPR] = 0u;
+#pragma unroll
+        for (int i = 0; i < L::WPR; ++i) out[i] = __builtin_amdgcn_alignbyte(raw[i + 1], raw[i], shift);
+    } else if (a.fmt == FMT_RG8) {
+        // normal map stored as RG8: 8 bytes at 8 bx (4x4) or 12 at 12 bx (6x6)
+        const uint32_t* w = reinterpret_cast<const uint32_t*>(row);
+#pragma unroll
+        for (int i = 0; i < L::WPR; ++i) out[i] = w[i];
+    } else {
+        // normal map in a three- or four-byte format: keep red and green
+        uint32_t t[DIM];
+        load_row_rgba<DIM>(a, row, t);
+#pragma unroll
+        for (int i = 0; i < DIM; i += 2) out[i / 2] = (t[i] & 0xFFFFu) | t[i + 1] << 16;
+    }
 }
 
-// Load a block's texels as packed RGBA8 words (loads only; no conversion), so
+// Load a block's texels in the class's layout (loads only; no conversion), so
 // that a caller can issue the next block's loads before encoding this one.
-template <int DIM>
-__device__ __forceinline__ void load_block(const FetchArgs& a, int bx, int by, uint32_t p[DIM * DIM]) {
+template <int DIM, int CH>
+__device__ __forceinline__ void load_block(const FetchArgs& a, int bx, int by, uint32_t p[Layout<DIM, CH>::WORDS]) {
+    using L = Layout<DIM, CH>;
     const int x0 = bx * DIM, y0 = by * DIM;
     const int bpp = bytes_per_texel(a.fmt);
     // Fast path: the block is inside the image and the 4-byte words covering its
@@ -166,21 +241,51 @@ __device__ __forceinline__ void load_block(const FetchArgs& a, int bx, int by, u
     const bool fast = a.aligned && (x0 + DIM <= a.width) && (y0 + DIM <= a.height) &&
                       (((x0 + DIM) * bpp + 3) & ~3) <= a.width * bpp;
     if (fast) {
+        const uint8_t* row = a.src + (size_t)(uint32_t)y0 * (uint32_t)a.row_pitch + uint32_t(x0 * bpp);
 #pragma unroll
-        for (int r = 0; r < DIM; ++r) load_row_fast<DIM>(a, x0, y0 + r, p + r * DIM);
+        for (int r = 0; r < DIM; ++r, row += a.row_pitch) load_row_fast<DIM, CH>(a, row, p + r * L::WPR);
     } else {
+        if (CH != CH_ALL) {
+#pragma unroll
+            for (int i = 0; i < L::WORDS; ++i) p[i] = 0u;
+        }
 #pragma unroll
         for (int r = 0; r < DIM; ++r)
 #pragma unroll
-            for (int i = 0; i < DIM; ++i) p[r * DIM + i] = load_texel_guarded(a, x0 + i, y0 + r);
+            for (int i = 0; i < DIM; ++i) {
+                const uint32_t t = load_texel_guarded(a, x0 + i, y0 + r);
+                if (CH == CH_ALL) {
+                    p[r * DIM + i] = t;
+                } else {
+#pragma unroll
+                    for (int c = 0; c < L::BPT; ++c) {
+                        const int j = i * L::BPT + c;
+                        p[r * L::WPR + (j >> 2)] |= ((t >> (8 * c)) & 0xFFu) << (8 * (j & 3));
+                    }
+                }
+            }
     }
 }
 
-template <int DIM, int NC = 4>
-__device__ __forceinline__ void unpack_block(const uint32_t p[DIM * DIM], uint32_t or_mask, const float* colour,
-                                             const float* unorm, float x[DIM * DIM][4]) {
+// Layout -> floats (x255 scale) through LDS tables: `colour` converts the colour
+// channels (sRGB or UNORM), `unorm` the alpha.  Only the class's channels are
+// converted (the others are known constants to the encoder, see encode_block).
+// or_mask (CH_ALL only) applies the normal-map override to the words.
+template <int DIM, int CH>
+__device__ __forceinline__ void unpack_block(const uint32_t p[Layout<DIM, CH>::WORDS], uint32_t or_mask,
+                                             const float* colour, const float* unorm, float x[DIM * DIM][4]) {
 #pragma unroll
-    for (int k = 0; k < DIM * DIM; ++k) unpack_texel<NC>(p[k] | or_mask, colour, unorm, x[k]);
+    for (int k = 0; k < DIM * DIM; ++k) {
+        if (CH == CH_ALL) {
+            const uint32_t w = p[k] | or_mask;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) x[k][c] = colour[(w >> (8 * c)) & 0xFFu];
+            x[k][3] = unorm[w >> 24];
+        } else {
+#pragma unroll
+            for (int c = 0; c < CH; ++c) x[k][c] = colour[layout_byte<DIM, CH>(p, k, c)];
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------

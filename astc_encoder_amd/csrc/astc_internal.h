@@ -70,7 +70,7 @@ constexpr int kMaxSegs = 16;  // images packed into one work item
 struct BatchLevel {   // one image (texture or mip level)
     FetchArgs f;
     uint32_t* dst;
-    int xblocks, nblocks;
+    int nblocks, pad;
 };
 struct BatchSeg { int level, first_block, count, pad; };
 struct BatchItem { int mode, seg_begin, nseg, pad; };

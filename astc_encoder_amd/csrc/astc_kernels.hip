@@ -18,7 +18,6 @@ namespace astc {
 
 struct PlainArgs {
     FetchArgs f;
-    int xblocks;
     int nblocks;
     uint32_t* dst;
     const DeviceTables* tables;
@@ -39,18 +38,11 @@ __device__ __forceinline__ void stage_tables(const DeviceTables* t, float* lut, 
     }
 }
 
-template <int DIM>
-__device__ __forceinline__ void load_block_id(const FetchArgs& f, int xblocks, int id, uint32_t p[DIM * DIM]) {
-    const int by = id / xblocks;
-    load_block<DIM>(f, id - by * xblocks, by, p);
+template <int DIM, int CH>
+__device__ __forceinline__ void load_block_id(const FetchArgs& f, int id, uint32_t p[Layout<DIM, CH>::WORDS]) {
+    const int by = block_row_of(f, id);
+    load_block<DIM, CH>(f, id - by * f.xblocks, by, p);
 }
-
-// Channel classes: which channels are 255 in every texel of every block.
-//   CH_ALL    any content: four channels;
-//   CH_RGB8   an RGB8 image tiled exactly by its blocks (no texel outside the
-//             image, where alpha would be 0): alpha is constant, three channels;
-//   CH_NORMAL normal maps (blue = alpha = 255, outside the image too): two.
-enum : int { CH_ALL = 4, CH_RGB8 = 3, CH_NORMAL = 2 };
 
 __host__ __device__ inline int channel_class(int fmt, uint32_t flags, int width, int height, int dim) {
     if (flags & FLAG_NORMAL) return CH_NORMAL;
@@ -66,8 +58,8 @@ __host__ __device__ inline int channel_class(int fmt, uint32_t flags, int width,
 // the mask is zero for every other image, so only the four-channel
 // max-accumulation search sets it.
 template <int DIM, int SRGB, int CH, bool OVERRIDE = false>
-__device__ __forceinline__ void unpack_texels(const uint32_t p[DIM * DIM], const FetchArgs& f, const float* lut,
-                                              float x[DIM * DIM][4]) {
+__device__ __forceinline__ void unpack_texels(const uint32_t p[Layout<DIM, CH>::WORDS], const FetchArgs& f,
+                                              const float* lut, float x[DIM * DIM][4]) {
     const float* colour = SRGB < 0 ? lut + ((f.flags & FLAG_SRGB) ? 0 : 256) : SRGB == 1 ? lut : lut + 256;
     unpack_block<DIM, CH>(p, OVERRIDE ? f.or_mask : 0u, colour, lut + 256, x);
 }
@@ -83,9 +75,9 @@ __device__ __forceinline__ void store_block(uint32_t* dst, int id, const Block12
 // current ones are converted (into the same registers), so the loads are in
 // flight while the block is encoded; without, after the block is stored.
 template <int DIM, bool ALPHA, int SRGB, int CH, int METHOD = METHOD_PCA>
-__device__ __forceinline__ void encode_loaded(const FetchArgs& f, int xblocks, uint32_t* dst, int first, int count,
-                                              bool prefetch, const float* lut, const PackTables* trit,
-                                              uint32_t cur[DIM * DIM]) {
+__device__ __forceinline__ void encode_loaded(const FetchArgs& f, uint32_t* dst, int first, int count, bool prefetch,
+                                              const float* lut, const PackTables* trit,
+                                              uint32_t cur[Layout<DIM, CH>::WORDS]) {
     const int stride = blockDim.x;
 #pragma unroll 1
     for (int o = threadIdx.x;; o += stride) {
@@ -93,20 +85,20 @@ __device__ __forceinline__ void encode_loaded(const FetchArgs& f, int xblocks, u
         float x[DIM * DIM][4];
         unpack_texels<DIM, SRGB, CH, METHOD == METHOD_MAX_ACCUMULATION>(cur, f, lut, x);
         // the packed words are dead once converted: the next block's loads reuse them
-        if (has_next && prefetch) load_block_id<DIM>(f, xblocks, first + o + stride, cur);
+        if (has_next && prefetch) load_block_id<DIM, CH>(f, first + o + stride, cur);
         store_block(dst, first + o, encode_block<DIM, ALPHA, CH, METHOD>(x, trit));
         if (!has_next) break;
-        if (!prefetch) load_block_id<DIM>(f, xblocks, first + o + stride, cur);
+        if (!prefetch) load_block_id<DIM, CH>(f, first + o + stride, cur);
     }
 }
 
 template <int DIM, bool ALPHA, int SRGB, int CH, int METHOD = METHOD_PCA>
-__device__ __forceinline__ void encode_chunk(const FetchArgs& f, int xblocks, uint32_t* dst, int first, int count,
-                                             bool prefetch, const float* lut, const PackTables* trit) {
+__device__ __forceinline__ void encode_chunk(const FetchArgs& f, uint32_t* dst, int first, int count, bool prefetch,
+                                             const float* lut, const PackTables* trit) {
     if (int(threadIdx.x) >= count) return;
-    uint32_t cur[DIM * DIM];
-    load_block_id<DIM>(f, xblocks, first + threadIdx.x, cur);
-    encode_loaded<DIM, ALPHA, SRGB, CH, METHOD>(f, xblocks, dst, first, count, prefetch, lut, trit, cur);
+    uint32_t cur[Layout<DIM, CH>::WORDS];
+    load_block_id<DIM, CH>(f, first + threadIdx.x, cur);
+    encode_loaded<DIM, ALPHA, SRGB, CH, METHOD>(f, dst, first, count, prefetch, lut, trit, cur);
 }
 
 // 4x4 blocks fit in 128 VGPRs (four waves per SIMD); 6x6 blocks hold 144
@@ -124,7 +116,7 @@ __attribute__((amdgpu_waves_per_eu(PlainWaves<DIM>::value))) encode_plain_kernel
     const int chunk = kPlainThreads * bpl;
     const int first = blockIdx.x * chunk;
     const int count = min(chunk, a.nblocks - first);
-    encode_chunk<DIM, ALPHA, SRGB ? 1 : 0, CH, METHOD>(a.f, a.xblocks, a.dst, first, count, true, lut, &trit);
+    encode_chunk<DIM, ALPHA, SRGB ? 1 : 0, CH, METHOD>(a.f, a.dst, first, count, true, lut, &trit);
 }
 
 // ---------------------------------------------------------------------------
@@ -144,8 +136,7 @@ __device__ __forceinline__ void process_item(const BatchArgs& a, const BatchItem
     if (item.nseg == 1) {
         const BatchSeg sg = a.segs[item.seg_begin];
         const BatchLevel& L = a.levels[sg.level];
-        encode_chunk<DIM, ALPHA, -1, CH>(L.f, L.xblocks, L.dst, sg.first_block, sg.count, a.prefetch != 0, lut,
-                                         trit);
+        encode_chunk<DIM, ALPHA, -1, CH>(L.f, L.dst, sg.first_block, sg.count, a.prefetch != 0, lut, trit);
         return;
     }
     int o = threadIdx.x;
@@ -153,8 +144,8 @@ __device__ __forceinline__ void process_item(const BatchArgs& a, const BatchItem
         const BatchSeg sg = a.segs[item.seg_begin + s];
         if (o < sg.count) {
             const BatchLevel& L = a.levels[sg.level];
-            uint32_t p[DIM * DIM];
-            load_block_id<DIM>(L.f, L.xblocks, sg.first_block + o, p);
+            uint32_t p[Layout<DIM, CH>::WORDS];
+            load_block_id<DIM, CH>(L.f, sg.first_block + o, p);
             float x[DIM * DIM][4];
             unpack_texels<DIM, -1, CH>(p, L.f, lut, x);
             store_block(L.dst, sg.first_block + o, encode_block<DIM, ALPHA, CH>(x, trit));
@@ -322,7 +313,7 @@ hipError_t launch_encode_plain(const DeviceTables* d_tables, const astc_image_de
     a.f.flags = d.flags;
     a.f.aligned = ((reinterpret_cast<uintptr_t>(src) & 3u) == 0) && ((d.row_pitch & 3) == 0);
     a.f.or_mask = normal_or_mask(d.flags);
-    a.xblocks = g.xblocks;
+    set_block_row(&a.f, g.xblocks);
     a.nblocks = int(g.nblocks);
     a.dst = static_cast<uint32_t*>(dst);
     a.tables = d_tables;

@@ -194,8 +194,8 @@ int geometry(const astc_request& r, ReqGeom* g) {
         L.f.flags = r.images[i].flags;
         L.f.aligned = (r.images[i].row_pitch & 3) == 0;
         L.f.or_mask = normal_or_mask(r.images[i].flags);
+        set_block_row(&L.f, ig.xblocks);
         L.dst = reinterpret_cast<uint32_t*>(oo);
-        L.xblocks = ig.xblocks;
         L.nblocks = (int)ig.nblocks;
         g->modes[i] = mode_of(r.images[i].block_dim, r.images[i].src_format, r.images[i].flags, r.images[i].width, r.images[i].height);
         so += align_up((int64_t)r.images[i].row_pitch * r.images[i].height);
