@@ -161,20 +161,15 @@ __device__ __forceinline__ uint32_t load_texel_guarded(const FetchArgs& a, int x
     return uint32_t(p[0]) | uint32_t(p[1]) << 8 | 0xFF000000u;
 }
 
-// One texel row as packed RGBA8 words, from 4-byte-aligned words of a row that
-// starts at `row` (any source format).
+// One texel row of an RGB8 or RG8 source as packed RGBA8 words, from
+// 4-byte-aligned words around a row that starts at `row`.
 template <int DIM>
-__device__ __forceinline__ void load_row_rgba(const FetchArgs& a, const uint8_t* row, uint32_t out[DIM]) {
+__device__ __forceinline__ void load_row_expand(const FetchArgs& a, const uint8_t* row, uint32_t out[DIM]) {
+    // the row's bytes start `shift` bytes into the first word; the words are
+    // realigned with a funnel shift so that byte j sits at a fixed place
     const int bpp = bytes_per_texel(a.fmt);
     const uint32_t shift = uint32_t(reinterpret_cast<uintptr_t>(row) & 3u);
     const uint32_t* w = reinterpret_cast<const uint32_t*>(row - shift);
-    if (a.fmt == FMT_RGBA8) {
-#pragma unroll
-        for (int i = 0; i < DIM; ++i) out[i] = w[i];
-        return;
-    }
-    // RGB8 / RG8: the row's bytes start `shift` bytes into the first word; the
-    // words are realigned with a funnel shift so that byte j sits at a fixed place.
     constexpr int kWords = (DIM * 3 + 3) / 4 + 1;
     const int nwords = int((DIM * bpp + shift + 3) >> 2);
     uint32_t raw[kWords + 1];
@@ -194,38 +189,58 @@ __device__ __forceinline__ void load_row_rgba(const FetchArgs& a, const uint8_t*
     }
 }
 
-// One texel row of a fully in-bounds block in the class's layout.
+// The texel rows of a fully in-bounds block whose first row starts at `row`,
+// in the class's layout.  The source format is tested once per block.
 template <int DIM, int CH>
-__device__ __forceinline__ void load_row_fast(const FetchArgs& a, const uint8_t* row, uint32_t* out) {
+__device__ __forceinline__ void load_rows_fast(const FetchArgs& a, const uint8_t* row, uint32_t* out) {
     using L = Layout<DIM, CH>;
-    if (CH == CH_ALL) {
-        load_row_rgba<DIM>(a, row, out);
-    } else if (CH == CH_RGB8 && DIM == 4) {
+    const int pitch = a.row_pitch;
+    if (CH == CH_RGB8 && DIM == 4) {
         // 12 bytes at 12 bx past an aligned row start: three aligned words
-        const uint32_t* w = reinterpret_cast<const uint32_t*>(row);
 #pragma unroll
-        for (int i = 0; i < L::WPR; ++i) out[i] = w[i];
+        for (int r = 0; r < DIM; ++r, row += pitch)
+#pragma unroll
+            for (int i = 0; i < L::WPR; ++i) out[r * L::WPR + i] = reinterpret_cast<const uint32_t*>(row)[i];
     } else if (CH == CH_RGB8) {
         // 18 bytes at 18 bx: they start 0 or 2 bytes into an aligned word
         const uint32_t shift = uint32_t(reinterpret_cast<uintptr_t>(row) & 3u);
-        const uint32_t* w = reinterpret_cast<const uint32_t*>(row - shift);
-        uint32_t raw[L::WPR + 1];
+        row -= shift;
 #pragma unroll
-        for (int i = 0; i < L::WPR; ++i) raw[i] = w[i];
-        raw[L::WPR] = 0u;
+        for (int r = 0; r < DIM; ++r, row += pitch) {
+            const uint32_t* w = reinterpret_cast<const uint32_t*>(row);
+            uint32_t raw[L::WPR + 1];
 #pragma unroll
-        for (int i = 0; i < L::WPR; ++i) out[i] = __builtin_amdgcn_alignbyte(raw[i + 1], raw[i], shift);
-    } else if (a.fmt == FMT_RG8) {
-        // normal map stored as RG8: 8 bytes at 8 bx (4x4) or 12 at 12 bx (6x6)
-        const uint32_t* w = reinterpret_cast<const uint32_t*>(row);
+            for (int i = 0; i < L::WPR; ++i) raw[i] = w[i];
+            raw[L::WPR] = 0u;
 #pragma unroll
-        for (int i = 0; i < L::WPR; ++i) out[i] = w[i];
+            for (int i = 0; i < L::WPR; ++i) out[r * L::WPR + i] = __builtin_amdgcn_alignbyte(raw[i + 1], raw[i], shift);
+        }
+    } else if (a.fmt == (CH == CH_NORMAL ? FMT_RG8 : FMT_RGBA8)) {
+        // the source bytes are the layout: RGBA8 words, or RG8 pairs of a normal
+        // map (8 bytes at 8 bx, 12 at 12 bx), at aligned addresses
+#pragma unroll
+        for (int r = 0; r < DIM; ++r, row += pitch)
+#pragma unroll
+            for (int i = 0; i < L::WPR; ++i) out[r * L::WPR + i] = reinterpret_cast<const uint32_t*>(row)[i];
     } else {
-        // normal map in a three- or four-byte format: keep red and green
-        uint32_t t[DIM];
-        load_row_rgba<DIM>(a, row, t);
 #pragma unroll
-        for (int i = 0; i < DIM; i += 2) out[i / 2] = (t[i] & 0xFFFFu) | t[i + 1] << 16;
+        for (int r = 0; r < DIM; ++r, row += pitch) {
+            uint32_t t[DIM];
+            if (a.fmt == FMT_RGBA8) {
+#pragma unroll
+                for (int i = 0; i < DIM; ++i) t[i] = reinterpret_cast<const uint32_t*>(row)[i];
+            } else {
+                load_row_expand<DIM>(a, row, t);
+            }
+            if (CH == CH_ALL) {
+#pragma unroll
+                for (int i = 0; i < DIM; ++i) out[r * DIM + i] = t[i];
+            } else {
+                // normal map in a three- or four-byte format: keep red and green
+#pragma unroll
+                for (int i = 0; i < DIM; i += 2) out[r * L::WPR + i / 2] = (t[i] & 0xFFFFu) | t[i + 1] << 16;
+            }
+        }
     }
 }
 
@@ -241,9 +256,7 @@ __device__ __forceinline__ void load_block(const FetchArgs& a, int bx, int by, u
     const bool fast = a.aligned && (x0 + DIM <= a.width) && (y0 + DIM <= a.height) &&
                       (((x0 + DIM) * bpp + 3) & ~3) <= a.width * bpp;
     if (fast) {
-        const uint8_t* row = a.src + (size_t)(uint32_t)y0 * (uint32_t)a.row_pitch + uint32_t(x0 * bpp);
-#pragma unroll
-        for (int r = 0; r < DIM; ++r, row += a.row_pitch) load_row_fast<DIM, CH>(a, row, p + r * L::WPR);
+        load_rows_fast<DIM, CH>(a, a.src + (size_t)(uint32_t)y0 * (uint32_t)a.row_pitch + uint32_t(x0 * bpp), p);
     } else {
         if (CH != CH_ALL) {
 #pragma unroll
