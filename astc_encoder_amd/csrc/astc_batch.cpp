@@ -36,16 +36,18 @@ namespace astc {
 // into chunks of 256 x blocks_per_lane blocks, images under 256 blocks packed
 // up to kMaxSegs per item (one block per thread).
 void plan_work(const std::vector<BatchLevel>& levels, const std::vector<int>& modes, int blocks_per_lane,
-               std::vector<BatchSeg>* segs, std::vector<BatchItem>* items, int* first4) {
+               std::vector<BatchSeg>* segs, std::vector<BatchItem>* items, int* first_light) {
     segs->clear();
     items->clear();
-    *first4 = -1;
+    *first_light = -1;
     const int chunk = kPlainThreads * blocks_per_lane;
     int order[MODE_COUNT];
     for (int m = 0; m < MODE_COUNT; ++m) order[m] = m;
     std::stable_sort(order, order + MODE_COUNT, [](int a, int b) { return mode_cost_rank(a) > mode_cost_rank(b); });
     for (int mode : order) {
-        if (!(mode & 2) && *first4 < 0) *first4 = (int)items->size();  // first 4x4 mode
+        // split point: the first 4x4 item of a reduced-channel class (4x4
+        // four-channel items rank above them and go with the 6x6 items)
+        if (!(mode & 2) && (mode & MODE_CH_MASK) != MODE_CH_ALL && *first_light < 0) *first_light = (int)items->size();
         std::vector<int> idx;
         for (int i = 0; i < (int)levels.size(); ++i)
             if (modes[i] == mode) idx.push_back(i);
@@ -81,7 +83,7 @@ void plan_work(const std::vector<BatchLevel>& levels, const std::vector<int>& mo
         }
         flush();
     }
-    if (*first4 < 0) *first4 = (int)items->size();
+    if (*first_light < 0) *first_light = (int)items->size();
 }
 
 }  // namespace astc
@@ -103,8 +105,8 @@ int build(astc_batch* b) {
     free_worklist(b);
     std::vector<BatchSeg> segs;
     std::vector<BatchItem> items;
-    int first4 = 0;
-    plan_work(b->levels, b->modes, b->tuning.blocks_per_lane, &segs, &items, &first4);
+    int first_light = 0;
+    plan_work(b->levels, b->modes, b->tuning.blocks_per_lane, &segs, &items, &first_light);
     hipError_t e;
     if ((e = hipMalloc(&b->d_segs, std::max<size_t>(1, segs.size()) * sizeof(BatchSeg))) != hipSuccess)
         return hip_fail(e, "hipMalloc(segs)");
@@ -117,7 +119,7 @@ int build(astc_batch* b) {
                                          hipMemcpyHostToDevice)) != hipSuccess)
         return hip_fail(e, "hipMemcpy(items)");
     b->nitems = (int)items.size();
-    int scal[4] = {b->nitems, first4, 0, 0};
+    int scal[4] = {b->nitems, first_light, 0, 0};
     if ((e = hipMemcpy(b->d_scalars, scal, sizeof(scal), hipMemcpyHostToDevice)) != hipSuccess)
         return hip_fail(e, "hipMemcpy(scalars)");
 
@@ -131,7 +133,7 @@ int build(astc_batch* b) {
     a.prefetch = b->tuning.prefetch;
     BatchLaunch l;
     l.grid = std::max(1, b->num_cus * b->tuning.workgroups_per_cu);
-    l.grid4 = 2 * l.grid;
+    l.grid4 = (5 * l.grid + 1) / 2;  // the 4x4 kernel holds five waves per SIMD to the 6x6 kernel's two
     l.split = b->tuning.split_modes;
     l.side = b->side_stream;
     l.fork = b->fork;

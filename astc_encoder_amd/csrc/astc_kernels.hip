@@ -165,12 +165,30 @@ __device__ __forceinline__ void process_item_ch(const BatchArgs& a, const BatchI
     }
 }
 
-// SEL = 0: every item (one kernel holds all four block-size x alpha paths and
-// the 6x6 register budget, two waves per SIMD); SEL = 6 / 4: only the 6x6 items
-// (the front of the list) or only the 4x4 items, each with its own register
-// budget (the 4x4 kernel runs at four waves per SIMD).
+// SEL = 0: every item (one kernel holds every path and the 6x6 register
+// budget, two waves per SIMD).  SEL = 6 / 4 split the list at the first 4x4
+// item of a reduced-channel class (RGB8 tiles, normal maps): the 6x6 kernel
+// takes the front (6x6 items, then 4x4 four-channel items, which fit its
+// budget), the 4x4 kernel the rest, with the smaller budget of those two
+// classes (five waves per SIMD).
 template <int SEL>
-struct BatchWaves { static constexpr int value = SEL == 4 ? 4 : 2; };
+struct BatchWaves { static constexpr int value = SEL == 4 ? 5 : 2; };
+
+template <int SEL, bool ALPHA>
+__device__ __forceinline__ void process_item_4x4(const BatchArgs& a, const BatchItem& item, const float* lut,
+                                                 const PackTables* trit) {
+    switch (item.mode & MODE_CH_MASK) {
+    case MODE_CH_NORMAL:
+        if (SEL != 6) process_item<4, ALPHA, CH_NORMAL>(a, item, lut, trit);
+        break;
+    case MODE_CH_RGB8:
+        if (SEL != 6) process_item<4, ALPHA, CH_RGB8>(a, item, lut, trit);
+        break;
+    default:
+        if (SEL != 4) process_item<4, ALPHA, CH_ALL>(a, item, lut, trit);
+        break;
+    }
+}
 
 template <int SEL>
 __global__ void __launch_bounds__(kPlainThreads) __attribute__((amdgpu_waves_per_eu(BatchWaves<SEL>::value)))
@@ -179,9 +197,9 @@ encode_batch_kernel(BatchArgs a) {
     __shared__ PackTables trit;
     __shared__ int next_item;
     stage_tables(a.tables, lut, &trit);
-    const int nitems = a.scalars[0], first4 = a.scalars[1];
-    const int begin = SEL == 4 ? first4 : 0;
-    const int end = SEL == 6 ? first4 : nitems;
+    const int nitems = a.scalars[0], first_light = a.scalars[1];
+    const int begin = SEL == 4 ? first_light : 0;
+    const int end = SEL == 6 ? first_light : nitems;
     int* counter = a.counters + (SEL == 4 ? 1 : 0);
     // Thread 0 claims the workgroup's next item while the current one is being
     // encoded, so the atomic's round trip is off the critical path.
@@ -195,15 +213,15 @@ encode_batch_kernel(BatchArgs a) {
         if (it >= end) break;
         if (threadIdx.x == 0) claimed = atomicAdd(counter, 1);
         const BatchItem item = a.items[it];
-        // one code path per block size x alpha; linear and sRGB fetches share
-        // it through the table choice (separate specialisations would not fit
-        // the register budget)
+        // one code path per block size x alpha x channel class; linear and sRGB
+        // fetches share it through the table choice (separate specialisations
+        // would not fit the register budget)
         switch (item.mode & 3) {
         case MODE_4x4_RGB:
-            if (SEL != 6) process_item_ch<4, false>(a, item, lut, &trit);
+            process_item_4x4<SEL, false>(a, item, lut, &trit);
             break;
         case MODE_4x4_RGBA:
-            if (SEL != 6) process_item_ch<4, true>(a, item, lut, &trit);
+            process_item_4x4<SEL, true>(a, item, lut, &trit);
             break;
         case MODE_6x6_RGB:
             if (SEL != 4) process_item_ch<6, false>(a, item, lut, &trit);

@@ -24,7 +24,7 @@ using namespace astc;
 
 namespace astc {
 void plan_work(const std::vector<BatchLevel>& levels, const std::vector<int>& modes, int blocks_per_lane,
-               std::vector<BatchSeg>* segs, std::vector<BatchItem>* items, int* first4);
+               std::vector<BatchSeg>* segs, std::vector<BatchItem>* items, int* first_light);
 }
 
 namespace {
@@ -156,7 +156,7 @@ int ensure_capacity(astc_server* sv, Slot& s, size_t nlev, size_t nseg, size_t n
     a.prefetch = sv->cfg.tuning.prefetch;
     BatchLaunch l;
     l.grid = std::max(1, sv->num_cus * sv->cfg.tuning.workgroups_per_cu);
-    l.grid4 = 2 * l.grid;
+    l.grid4 = (5 * l.grid + 1) / 2;  // the 4x4 kernel holds five waves per SIMD to the 6x6 kernel's two
     l.split = sv->cfg.tuning.split_modes;
     l.side = s.side;
     l.fork = s.fork;
@@ -394,8 +394,8 @@ int astc_server_run_trace(astc_server* sv, const astc_request* reqs, int n, doub
                     }
                 }
                 if (s.reqs.empty()) break;  // the ring is full: wait for completions
-                int first4 = 0;
-                plan_work(levels, modes, sv->cfg.tuning.blocks_per_lane, &segs, &items, &first4);
+                int first_light = 0;
+                plan_work(levels, modes, sv->cfg.tuning.blocks_per_lane, &segs, &items, &first_light);
                 if (levels.size() > s.cap_levels || segs.size() > s.cap_segs || items.size() > s.cap_items) {
                     // grow every idle slot at once: regrowing (allocation + graph capture)
                     // costs milliseconds, so it should not recur slot by slot mid-stream
@@ -409,7 +409,7 @@ int astc_server_run_trace(astc_server* sv, const astc_request* reqs, int n, doub
                 std::memcpy(s.h_segs, segs.data(), segs.size() * sizeof(BatchSeg));
                 std::memcpy(s.h_items, items.data(), items.size() * sizeof(BatchItem));
                 s.h_scalars[0] = (int)items.size();
-                s.h_scalars[1] = first4;
+                s.h_scalars[1] = first_light;
                 s.h_scalars[2] = 0;
                 s.h_scalars[3] = 0;
                 e = hipSuccess;

@@ -97,6 +97,9 @@ def test_batch_with_largest_dimensions(enc):
                                              (1, 1, 0, 1), (2, 2, 1, 1), (4, 2, 1, 1), (8, 4, 0, 1)])
 def test_every_tuning_gives_the_same_blocks(enc, bpl, wpc, pf, split):
     images = mixed_images(enc, 2, 24)
+    # four-channel 4x4 images (which the split form runs in the 6x6 kernel)
+    raw = torch.from_numpy(np.random.default_rng(bpl * 10 + wpc).integers(0, 256, (75, 61, 4), dtype=np.uint8)).cuda()
+    images += [(raw, 4, astc.FLAG_ALPHA), (raw[:, :, :3].contiguous(), 4, 0), (raw[:9, :7], 4, astc.FLAG_SRGB)]
     want = plain_outputs(enc, images)
     b = astc.Batch(enc, images)
     b.set_tuning(bpl, wpc, pf, split)
