@@ -101,8 +101,9 @@ __device__ __forceinline__ void encode_chunk(const FetchArgs& f, uint32_t* dst, 
     encode_loaded<DIM, ALPHA, SRGB, CH, METHOD>(f, dst, first, count, prefetch, lut, trit, cur);
 }
 
-// 4x4 blocks fit in 128 VGPRs (four waves per SIMD); 6x6 blocks hold 144
-// texel floats and run at two.
+// Minimum waves per SIMD the register budget must allow: 4x4 encoders fit in
+// 128 VGPRs (the reduced-channel ones use 68-90 and run at five to seven waves);
+// 6x6 blocks hold 144 texel floats and run at two.
 template <int DIM>
 struct PlainWaves { static constexpr int value = DIM == 4 ? 4 : 2; };
 
