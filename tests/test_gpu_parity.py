@@ -93,7 +93,7 @@ def test_channel_classes(enc, dim, channels, flags, shape):
 
 
 @pytest.mark.parametrize("dim", [4, 6])
-@pytest.mark.parametrize("channels,flags", [(4, 1), (4, 5), (3, 0), (3, 4), (2, 2), (4, 0)])
+@pytest.mark.parametrize("channels,flags", [(4, 1), (4, 5), (3, 0), (3, 4), (2, 2), (4, 0), (4, 2), (3, 2), (4, 3)])
 @pytest.mark.parametrize("shape", [(36, 48), (29, 53)])
 def test_max_accumulation_search(enc, dim, channels, flags, shape):
     """The optional endpoint search (ASTC_Encode.hlsl:170-227) against the oracle's."""
