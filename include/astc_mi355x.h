@@ -140,8 +140,9 @@ typedef struct astc_tuning {
     int32_t blocks_per_lane;   /* 1, 2, 4 or 8: a work item is 256 x this many blocks of one image */
     int32_t workgroups_per_cu; /* persistent 256-thread workgroups per compute unit, 1..4 */
     int32_t prefetch;          /* 1: load the next block's texels while encoding the current one */
-    int32_t split_modes;       /* 0: one kernel for every item; 1: a 6x6 kernel and a 4x4 kernel
-                                  (own register budgets), the 4x4 one on a second stream */
+    int32_t split_modes;       /* 0: one kernel for every item; 1: two kernels with their own
+                                  register budgets, one for 6x6 and four-channel 4x4 items and
+                                  one, on a second stream, for 4x4 RGB8 and normal-map items */
 } astc_tuning;
 
 /* descs/srcs/dsts: `count` images (device pointers as for astc_encode).  The
