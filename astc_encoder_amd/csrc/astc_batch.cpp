@@ -133,7 +133,10 @@ int build(astc_batch* b) {
     a.prefetch = b->tuning.prefetch;
     BatchLaunch l;
     l.grid = std::max(1, b->num_cus * b->tuning.workgroups_per_cu);
-    l.grid4 = (5 * l.grid + 1) / 2;  // the 4x4 kernel holds five waves per SIMD to the 6x6 kernel's two
+    // the 4x4 kernel's own occupancy (five waves per SIMD), whatever the 6x6
+    // kernel's: with one 6x6 workgroup per CU, three 4x4 waves share each SIMD
+    // with the 6x6 wave from the start and cover its latency
+    l.grid4 = std::max(1, 5 * b->num_cus);
     l.split = b->tuning.split_modes;
     l.side = b->side_stream;
     l.fork = b->fork;

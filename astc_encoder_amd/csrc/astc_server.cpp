@@ -156,7 +156,10 @@ int ensure_capacity(astc_server* sv, Slot& s, size_t nlev, size_t nseg, size_t n
     a.prefetch = sv->cfg.tuning.prefetch;
     BatchLaunch l;
     l.grid = std::max(1, sv->num_cus * sv->cfg.tuning.workgroups_per_cu);
-    l.grid4 = (5 * l.grid + 1) / 2;  // the 4x4 kernel holds five waves per SIMD to the 6x6 kernel's two
+    // the 4x4 kernel's own occupancy (five waves per SIMD), whatever the 6x6
+    // kernel's: with one 6x6 workgroup per CU, three 4x4 waves share each SIMD
+    // with the 6x6 wave from the start and cover its latency
+    l.grid4 = std::max(1, 5 * sv->num_cus);
     l.split = sv->cfg.tuning.split_modes;
     l.side = s.side;
     l.fork = s.fork;

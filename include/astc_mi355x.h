@@ -138,7 +138,8 @@ typedef struct astc_batch astc_batch;
 
 typedef struct astc_tuning {
     int32_t blocks_per_lane;   /* 1, 2, 4 or 8: a work item is 256 x this many blocks of one image */
-    int32_t workgroups_per_cu; /* persistent 256-thread workgroups per compute unit, 1..4 */
+    int32_t workgroups_per_cu; /* persistent 256-thread workgroups per compute unit, 1..4 (with
+                                  split_modes, of the first kernel; the second runs five) */
     int32_t prefetch;          /* 1: load the next block's texels while encoding the current one */
     int32_t split_modes;       /* 0: one kernel for every item; 1: two kernels with their own
                                   register budgets, one for 6x6 and four-channel 4x4 items and
