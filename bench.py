@@ -231,43 +231,45 @@ def serving_sweep(enc, rates=(1000, 2000, 5000, 10000, 20000, 50000), seconds=0.
             "hbm_budget_gb": 64, "max_inflight_batches": 3, "sweep": results}
 
 
-def cpu_baseline(wl, img, gpu_blocks, budget_s=20.0):
+def _time_runs(fn, min_s, min_runs=5, max_runs=2000):
+    """Run fn (after one warm-up) until min_s seconds and min_runs runs have passed;
+    return the per-run times."""
+    fn()
+    times = []
+    start = time.perf_counter()
+    while len(times) < max_runs and (len(times) < min_runs or time.perf_counter() - start < min_s):
+        t = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t)
+    return times
+
+
+def cpu_baseline(wl, img, gpu_blocks, min_s=10.0):
     """The oracle (OpenMP over blocks, all cores of this host) on the same texture:
-    one warm-up, then the median of up to 5 timed runs within the budget."""
+    one warm-up, then whole-texture runs for at least min_s seconds; the median run."""
     from tests import oracle_ref as orc
     cores = len(os.sched_getaffinity(0))
-    t = time.perf_counter()
     ref = orc.oracle_encode(img, wl["dim"], wl["flags"], threads=cores)
-    first = time.perf_counter() - t
-    times = []
-    for _ in range(5):
-        if sum(times) + first > budget_s:
-            break
-        t = time.perf_counter()
-        orc.oracle_encode(img, wl["dim"], wl["flags"], threads=cores)
-        times.append(time.perf_counter() - t)
-    med = float(np.median(times)) if times else first
+    times = _time_runs(lambda: orc.oracle_encode(img, wl["dim"], wl["flags"], threads=cores), min_s)
+    med = float(np.median(times))
     match = float(np.mean(np.all(ref == gpu_blocks, axis=1)))
     # the oracle's own blocks through the oracle's spec decoder: the reference PSNR
     _, dec = orc.decode_image(ref, wl["w"], wl["h"], wl["dim"])
     return dict(value=round(wl["w"] * wl["h"] / med / 1e6, 2), unit="Mtexels/s", cores=cores, kind="port",
-                sample=f"whole texture ({wl['name']}), median of {max(1, len(times))} runs after 1 warm-up",
+                sample=f"whole texture ({wl['name']}), median of {len(times)} runs over {sum(times):.1f} s "
+                       f"after 1 warm-up",
                 gpu_block_match=match, oracle_psnr_db=round(orc.psnr(img, wl["flags"], dec), 4))
 
 
-def cpu_small_case(wl):
-    """The oracle on the 1024x1024 RGB8 4x4 texture (all host cores, median of 5)."""
+def cpu_small_case(wl, min_s=3.0):
+    """The oracle on the 1024x1024 RGB8 4x4 texture (all host cores; median run over
+    at least min_s seconds)."""
     from tests import oracle_ref as orc
     cores = len(os.sched_getaffinity(0))
     img = wl["gen"](0, wl["w"], wl["h"])
-    orc.oracle_encode(img, wl["dim"], wl["flags"], threads=cores)
-    times = []
-    for _ in range(5):
-        t = time.perf_counter()
-        orc.oracle_encode(img, wl["dim"], wl["flags"], threads=cores)
-        times.append(time.perf_counter() - t)
+    times = _time_runs(lambda: orc.oracle_encode(img, wl["dim"], wl["flags"], threads=cores), min_s)
     return {"workload": wl["name"], "mtexels_s": round(wl["w"] * wl["h"] / float(np.median(times)) / 1e6, 1),
-            "cores": cores}
+            "cores": cores, "runs": len(times)}
 
 
 def main():
