@@ -264,9 +264,11 @@ int astc_server_create(astc_ctx* ctx, const astc_server_config* cfg, astc_server
                     rc = hip_fail(e, "slot setup");
                     break;
                 }
-                // work-list room for a batch of ~1 G texels (2 MB per slot), so that
-                // regrowing (reallocation + graph capture, milliseconds) stays rare
-                if ((rc = ensure_capacity(sv, s, 1024, 65536, 65536)) != ASTC_OK) break;
+                // work-list room for a full batch of mip-chained textures (16 levels
+                // per request) and ~1 G texels, so that regrowing (reallocation and
+                // graph capture, milliseconds) stays rare
+                const size_t levels_room = std::max<size_t>(1024, 16 * size_t(cfg->max_batch_requests));
+                if ((rc = ensure_capacity(sv, s, levels_room, 65536, 65536)) != ASTC_OK) break;
             }
         }
         // Replay every slot's graph once on an empty work list (the kernels exit at
