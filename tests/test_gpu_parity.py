@@ -132,6 +132,43 @@ def test_unaligned_source_and_padded_pitch(enc, channels, dim):
     assert_same(out.cpu().numpy(), orc.oracle_encode(img, dim, astc.FLAG_ALPHA), "unaligned")
 
 
+def random_case(seed):
+    """A seeded random encode: size, format, block size, flags (any combination,
+    the optional search included), row padding and source misalignment, with
+    texels drawn from a mix of noise, flat runs and two-level patterns."""
+    rng = np.random.default_rng(1000 + seed)
+    w, h = int(rng.integers(1, 200)), int(rng.integers(1, 200))
+    channels = int(rng.choice([4, 3, 2]))
+    dim = int(rng.choice([4, 6]))
+    flags = int(rng.integers(0, 16))
+    pad = int(rng.choice([0, 0, 1, 4, 13]))
+    shift = int(rng.choice([0, 0, 0, 1, 2, 3]))
+    img = rng.integers(0, 256, size=(h, w, channels), dtype=np.uint8)
+    kind = int(rng.integers(0, 3))
+    if kind == 1:
+        img[:] = np.repeat(rng.integers(0, 256, size=(h, 1, channels), dtype=np.uint8), w, axis=1)
+    elif kind == 2:
+        img[:] = np.where(rng.random((h, w, 1)) < 0.5, rng.integers(0, 256, channels), rng.integers(0, 256, channels))
+    return w, h, channels, dim, flags, pad, shift, img
+
+
+@pytest.mark.parametrize("seed", range(48))
+def test_random_cases(enc, seed):
+    w, h, channels, dim, flags, pad, shift, img = random_case(seed)
+    print(f"case seed={seed} {w}x{h} ch={channels} dim={dim} flags={flags} pad={pad} shift={shift}", flush=True)
+    pitch = w * channels + pad
+    buf = np.zeros(h * pitch + shift + 16, dtype=np.uint8)
+    for y in range(h):
+        buf[shift + y * pitch: shift + y * pitch + w * channels] = img[y].reshape(-1)
+    dev = torch.from_numpy(buf).cuda()
+    fmt = {4: astc.FMT_RGBA8, 3: astc.FMT_RGB8, 2: astc.FMT_RG8}[channels]
+    desc = astc.make_desc(w, h, fmt, dim, flags, row_pitch=pitch)
+    out = torch.empty((astc.output_size(desc) // 16, 16), dtype=torch.uint8, device="cuda")
+    enc.encode_raw(desc, dev.data_ptr() + shift, out.data_ptr())
+    torch.cuda.synchronize()
+    assert_same(out.cpu().numpy(), orc.oracle_encode(img, dim, flags), f"seed {seed}")
+
+
 WORKLOADS = [("sinusoid RGB8 1024x1024 4x4", synth.sinusoid_rgb8, 1024, 1024, 4, 0),
              ("noise RGBA8 sRGB 4096x4096 6x6", synth.noise_rgba8, 4096, 4096, 6, astc.FLAG_ALPHA | astc.FLAG_SRGB),
              ("normal RG8 2048x2048 4x4", synth.normal_rg8, 2048, 2048, 4, astc.FLAG_NORMAL)]
