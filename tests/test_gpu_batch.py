@@ -111,6 +111,29 @@ def test_every_tuning_gives_the_same_blocks(enc, bpl, wpc, pf, split):
     b.close()
 
 
+@pytest.mark.parametrize("seed", range(6))
+def test_random_batches(enc, seed):
+    """Seeded random batches: sizes, formats, block sizes and flags mixed freely
+    (some images as narrowed views, so with padded rows), under a random tuning."""
+    rng = np.random.default_rng(500 + seed)
+    images = []
+    for _ in range(int(rng.integers(5, 40))):
+        w, h = int(rng.integers(1, 300)), int(rng.integers(1, 300))
+        c = int(rng.choice([4, 3, 2]))
+        t = torch.from_numpy(rng.integers(0, 256, size=(h, w + int(rng.integers(0, 3)), c), dtype=np.uint8)).cuda()
+        images.append((t[:, :w], int(rng.choice([4, 6])), int(rng.integers(0, 8))))
+    want = plain_outputs(enc, images)
+    b = astc.Batch(enc, images)
+    tuning = (int(rng.choice([1, 2, 4, 8])), int(rng.integers(1, 3)), int(rng.integers(0, 2)), int(rng.integers(0, 2)))
+    print(f"case seed={seed} images={len(images)} tuning={tuning}", flush=True)
+    b.set_tuning(*tuning)
+    b.launch()
+    torch.cuda.synchronize()
+    for i, (o, w) in enumerate(zip(b.outputs, want)):
+        np.testing.assert_array_equal(o.cpu().numpy(), w, err_msg=f"image {i}")
+    b.close()
+
+
 @pytest.mark.parametrize("split", [0, 1])
 def test_graph_replays_with_eager_work_between(enc, split):
     images = mixed_images(enc, 3, 16)
