@@ -117,6 +117,37 @@ def time_plain(enc, wl, steps, warmup):
                 nblocks=nblocks, mtexels_s=texels * steps / wall / 1e6, psnr_db=gpu_psnr(enc, wl, src, out))
 
 
+def time_host_resident(enc, wl, steps):
+    """The headline texture starting in (pinned) host memory: host-to-device copy,
+    encode, blocks back to (pinned) host, per step, on one stream."""
+    import torch
+    img = wl["gen"](0, wl["w"], wl["h"])
+    host_src = torch.from_numpy(img).pin_memory()
+    dev_src = torch.empty(host_src.shape, dtype=torch.uint8, device="cuda")
+    out = enc.encode(dev_src.copy_(host_src), block_dim=wl["dim"], flags=wl["flags"])
+    host_out = torch.empty(out.shape, dtype=torch.uint8).pin_memory()
+    stream = torch.cuda.current_stream()
+
+    def step():
+        dev_src.copy_(host_src, non_blocking=True)
+        enc.encode(dev_src, block_dim=wl["dim"], flags=wl["flags"], out=out)
+        host_out.copy_(out, non_blocking=True)
+
+    for _ in range(3):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    return {"workload": wl["name"] + ", source in pinned host memory, blocks back to pinned host",
+            "ms_per_step": round(ms, 4), "mtexels_s": round(wl["w"] * wl["h"] / (ms * 1e-3) / 1e6, 1),
+            "h2d_bytes": int(img.nbytes), "d2h_bytes": int(out.numel())}
+
+
 def gpu_psnr(enc, wl, src, blocks):
     """PSNR of the encoded texture, decoded by the library's GPU decoder, against the
     encoder's view of the source (sRGB linearised; the mode's channels: RG for
@@ -349,6 +380,7 @@ def main():
                            "psnr_db": e["psnr_db"],
                            "valu_tflops": round(FLOPS_PER_BLOCK[wl["flops_key"]] * e["nblocks"] /
                                                 (e["kernel_ms"] * 1e-3) / 1e12, 2)}
+        extras["host_resident_headline"] = time_host_resident(enc, head, 20)
         if not args.no_batched:
             extras["mixed_textures_batched_vs_plain"] = batched_vs_plain(enc, max(5, args.steps // 5), 2)
         if not args.no_serving:
