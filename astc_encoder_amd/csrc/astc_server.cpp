@@ -265,9 +265,11 @@ int astc_server_create(astc_ctx* ctx, const astc_server_config* cfg, astc_server
                     break;
                 }
                 // work-list room for a full batch of mip-chained textures (16 levels
-                // per request) and ~1 G texels, so that regrowing (reallocation and
-                // graph capture, milliseconds) stays rare
-                const size_t levels_room = std::max<size_t>(1024, 16 * size_t(cfg->max_batch_requests));
+                // per request, at most 65536 levels up front) and ~1 G texels, so
+                // that regrowing (reallocation and graph capture, milliseconds)
+                // stays rare; larger batches still grow on demand
+                const size_t levels_room =
+                    std::min<size_t>(65536, std::max<size_t>(1024, 16 * size_t(cfg->max_batch_requests)));
                 if ((rc = ensure_capacity(sv, s, levels_room, 65536, 65536)) != ASTC_OK) break;
             }
         }
