@@ -46,8 +46,12 @@ void plan_work(const std::vector<BatchLevel>& levels, const std::vector<int>& mo
     std::stable_sort(order, order + MODE_COUNT, [](int a, int b) { return mode_cost_rank(a) > mode_cost_rank(b); });
     for (int mode : order) {
         // split point: the first 4x4 item of a reduced-channel class (4x4
-        // four-channel items rank above them and go with the 6x6 items)
-        if (!(mode & 2) && (mode & MODE_CH_MASK) != MODE_CH_ALL && *first_light < 0) *first_light = (int)items->size();
+        // four-channel items rank above them and go with the 6x6 items).  The
+        // class is named explicitly: codes with both class bits set are not
+        // modes (no image has them) and rank with the four-channel modes.
+        const int ch = mode & MODE_CH_MASK;
+        if (!(mode & 2) && (ch == MODE_CH_RGB8 || ch == MODE_CH_NORMAL) && *first_light < 0)
+            *first_light = (int)items->size();
         std::vector<int> idx;
         for (int i = 0; i < (int)levels.size(); ++i)
             if (modes[i] == mode) idx.push_back(i);

@@ -51,6 +51,15 @@ def mixed_images(enc, seed, n):
     return images
 
 
+def poison(b):
+    """Fill the batch's outputs with a sentinel, so that a block the launch does
+    not write cannot pass for the expected one (freed plain outputs of the same
+    size are often handed back to the batch by the allocator)."""
+    for o in b.outputs:
+        o.fill_(0xA5)
+    torch.cuda.synchronize()
+
+
 def plain_outputs(enc, images):
     outs = [enc.encode(t, block_dim=d, flags=f) for t, d, f in images]
     torch.cuda.synchronize()
@@ -66,6 +75,7 @@ def test_batch_matches_plain_and_oracle(enc):
     b = astc.Batch(enc, images)
     info = b.info()
     assert info["blocks"] == sum(len(w) for w in want)
+    poison(b)
     b.launch()
     torch.cuda.synchronize()
     for i, (o, w) in enumerate(zip(b.outputs, want)):
@@ -86,6 +96,7 @@ def test_batch_with_largest_dimensions(enc):
     images = [(wide, 4, astc.FLAG_ALPHA), (tall, 6, astc.FLAG_SRGB)] + mixed_images(enc, 6, 4)
     want = plain_outputs(enc, images)
     b = astc.Batch(enc, images)
+    poison(b)
     b.launch()
     torch.cuda.synchronize()
     for i, (o, w) in enumerate(zip(b.outputs, want)):
@@ -104,6 +115,7 @@ def test_every_tuning_gives_the_same_blocks(enc, bpl, wpc, pf, split):
     b = astc.Batch(enc, images)
     b.set_tuning(bpl, wpc, pf, split)
     assert b.tuning() == {"blocks_per_lane": bpl, "workgroups_per_cu": wpc, "prefetch": pf, "split_modes": split}
+    poison(b)
     b.launch()
     torch.cuda.synchronize()
     for o, w in zip(b.outputs, want):
@@ -127,6 +139,7 @@ def test_random_batches(enc, seed):
     tuning = (int(rng.choice([1, 2, 4, 8])), int(rng.integers(1, 3)), int(rng.integers(0, 2)), int(rng.integers(0, 2)))
     print(f"case seed={seed} images={len(images)} tuning={tuning}", flush=True)
     b.set_tuning(*tuning)
+    poison(b)
     b.launch()
     torch.cuda.synchronize()
     for i, (o, w) in enumerate(zip(b.outputs, want)):

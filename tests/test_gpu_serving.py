@@ -77,6 +77,37 @@ def test_alone_and_in_the_mix_give_the_same_blocks(enc, pool):
     srv.close()
 
 
+@pytest.mark.parametrize("tuning", [(8, 1, 0, 1), (2, 2, 1, 1), (4, 2, 1, 0)])
+def test_every_mode_class_under_each_tuning(enc, tuning):
+    """Requests of every mode class (6x6, four-channel 4x4 in linear, alpha and
+    sRGB forms, tiled RGB8 4x4, normal maps), each with its own random content so
+    that no request's blocks can stand in for another's, served in a mix under
+    split and single-kernel tunings; outputs start as a sentinel."""
+    rng = np.random.default_rng(sum(tuning))
+    kinds = [(6, 4, astc.FLAG_ALPHA | astc.FLAG_SRGB), (4, 3, 0), (4, 4, astc.FLAG_ALPHA), (4, 4, astc.FLAG_SRGB),
+             (4, 3, 0), (4, 2, astc.FLAG_NORMAL)]
+    reqs, wants = [], []
+    for k in range(48):
+        dim, c, flags = kinds[k % len(kinds)]
+        w = int(rng.integers(2, 40)) * 4 if k % len(kinds) == 4 else int(rng.integers(3, 150))
+        h = int(rng.integers(2, 40)) * 4 if k % len(kinds) == 4 else int(rng.integers(3, 150))
+        img = torch.from_numpy(rng.integers(0, 256, size=(h, w, c), dtype=np.uint8)).cuda()
+        levels = enc.mip_chain(img)
+        pk = astc.PackedImages(levels, dim, flags)
+        want = [enc.encode(l, block_dim=dim, flags=flags) for l in levels]
+        dst = torch.full((pk.out_bytes,), 0xA5, dtype=torch.uint8, device="cuda")
+        reqs.append((pk, float(k) * 2e-5, dst))
+        wants.append(want)
+    torch.cuda.synchronize()
+    srv = astc.Server(enc, 512 << 20, max_inflight=3, max_batch_requests=16, tuning=tuning)
+    lat, st = srv.run_trace(reqs)
+    torch.cuda.synchronize()
+    assert st["requests"] == len(reqs)
+    for (pk, _, dst), want in zip(reqs, wants):
+        check_request(pk, want, dst)
+    srv.close()
+
+
 def test_admission_waits_for_ring_space(enc, pool):
     biggest = max(p.buffer.numel() + p.out_bytes for p, _ in pool)
     budget = biggest + 4096
