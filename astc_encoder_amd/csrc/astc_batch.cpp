@@ -14,7 +14,16 @@ struct astc_batch {
     int device = 0;
     int num_cus = 0;
     std::vector<BatchLevel> levels;  // host copies
-    std::vector<int> modes;
+    std::vector<int> modes;          // kNoMode for images encoded by plain launches
+    // images with the max-accumulation search: one plain launch each, captured in
+    // the same graph after the persistent kernels (the work list has no such mode)
+    struct PlainImage {
+        astc_image_desc desc;
+        ImageGeom geom;
+        const void* src;
+        void* dst;
+    };
+    std::vector<PlainImage> plain;
     int64_t blocks = 0, texels = 0;
     astc_tuning tuning{4, 2, 1, 1};
     // device work list
@@ -30,6 +39,9 @@ struct astc_batch {
 };
 
 namespace astc {
+
+// Mode code of an image that is not in the work list (no code the planner visits).
+constexpr int kNoMode = -1;
 
 // Work list for a tuning.  Modes are ordered by cost (6x6 RGBA first) so the
 // longest items start first; within a mode, images by size, large ones split
@@ -148,6 +160,10 @@ int build(astc_batch* b) {
     if ((e = hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
         return hip_fail(e, "hipStreamBeginCapture");
     hipError_t le = launch_encode_batch(a, l, b->capture_stream);
+    for (size_t i = 0; i < b->plain.size() && le == hipSuccess; ++i) {
+        const astc_batch::PlainImage& p = b->plain[i];
+        le = launch_encode_plain(b->ctx->d_tables, p.desc, p.geom, p.src, p.dst, b->capture_stream, b->num_cus);
+    }
     e = hipStreamEndCapture(b->capture_stream, &b->graph);
     if (le != hipSuccess) return hip_fail(le, "batch launch capture");
     if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
@@ -184,8 +200,6 @@ int astc_batch_create(astc_ctx* ctx, const astc_image_desc* descs, const void* c
         for (int i = 0; i < count; ++i) {
             ImageGeom g;
             int rc = check_desc(&descs[i], &g);
-            if (rc == ASTC_OK && (descs[i].flags & ASTC_FLAG_MAX_ACCUMULATION))
-                rc = fail(ASTC_ERR_UNSUPPORTED, "the max-accumulation search is available in astc_encode only");
             if (rc != ASTC_OK || !srcs[i] || !dsts[i] || (reinterpret_cast<uintptr_t>(dsts[i]) & 15u)) {
                 delete b;
                 if (rc == ASTC_OK) rc = fail(ASTC_ERR_INVALID_ARG, "image " + std::to_string(i) + ": bad src/dst pointer");
@@ -205,6 +219,10 @@ int astc_batch_create(astc_ctx* ctx, const astc_image_desc* descs, const void* c
             L.dst = static_cast<uint32_t*>(dsts[i]);
             L.nblocks = (int)g.nblocks;
             b->modes[i] = mode_of(descs[i].block_dim, descs[i].src_format, descs[i].flags, descs[i].width, descs[i].height);
+            if (descs[i].flags & ASTC_FLAG_MAX_ACCUMULATION) {
+                b->modes[i] = kNoMode;
+                b->plain.push_back(astc_batch::PlainImage{descs[i], g, srcs[i], dsts[i]});
+            }
             b->blocks += g.nblocks;
             b->texels += (int64_t)descs[i].width * descs[i].height;
         }

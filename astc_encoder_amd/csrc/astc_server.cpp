@@ -185,7 +185,7 @@ int geometry(const astc_request& r, ReqGeom* g) {
         int rc = check_desc(&r.images[i], &ig);
         if (rc != ASTC_OK) return rc;
         if (r.images[i].flags & ASTC_FLAG_MAX_ACCUMULATION)
-            return fail(ASTC_ERR_UNSUPPORTED, "the max-accumulation search is available in astc_encode only");
+            return fail(ASTC_ERR_UNSUPPORTED, "the max-accumulation search is not available in serving (astc_encode and astc_batch have it)");
         if (r.images[i].row_pitch != r.images[i].width * ig.bpp)
             return fail(ASTC_ERR_INVALID_ARG, "request images must be tightly packed");
         BatchLevel& L = g->levels[i];

@@ -211,9 +211,50 @@ def test_autotune_picks_a_valid_tuning(enc):
     b.close()
 
 
+@pytest.mark.parametrize("split", [0, 1])
+def test_max_accumulation_images_in_a_batch(enc, split):
+    """Images with the max-accumulation search (plain launches inside the batch's
+    graph) among work-list images: every image equals its plain encode, over graph
+    replays, and the search's blocks equal the oracle's."""
+    rng = np.random.default_rng(40 + split)
+    images = mixed_images(enc, 8, 6)
+    for k, (c, dim, flags) in enumerate([(4, 4, astc.FLAG_ALPHA), (3, 6, astc.FLAG_SRGB), (2, 4, astc.FLAG_NORMAL),
+                                         (4, 6, astc.FLAG_ALPHA | astc.FLAG_SRGB), (3, 4, 0)]):
+        h, w = int(rng.integers(1, 90)), int(rng.integers(1, 90))
+        t = torch.from_numpy(rng.integers(0, 256, size=(h, w, c), dtype=np.uint8)).cuda()
+        images.insert(2 * k, (t, dim, flags | astc.FLAG_MAX_ACCUMULATION))
+    want = plain_outputs(enc, images)
+    b = astc.Batch(enc, images)
+    b.set_tuning(4, 2, 1, split)
+    assert b.info()["blocks"] == sum(len(w) for w in want)
+    for rep in range(3):
+        poison(b)
+        b.launch()
+        torch.cuda.synchronize()
+        for i, (o, w) in enumerate(zip(b.outputs, want)):
+            np.testing.assert_array_equal(o.cpu().numpy(), w, err_msg=f"replay {rep} image {i}")
+    for i, (t, d, f) in enumerate(images):
+        if f & astc.FLAG_MAX_ACCUMULATION and t.shape[0] * t.shape[1] < 4000:
+            np.testing.assert_array_equal(b.outputs[i].cpu().numpy(), orc.oracle_encode(t.cpu().numpy(), d, f))
+    b.close()
+
+
+def test_batch_of_max_accumulation_images_only(enc):
+    rng = np.random.default_rng(41)
+    images = [(torch.from_numpy(rng.integers(0, 256, size=(33, 47, 4), dtype=np.uint8)).cuda(), d,
+               astc.FLAG_ALPHA | astc.FLAG_MAX_ACCUMULATION) for d in (4, 6)]
+    want = plain_outputs(enc, images)
+    b = astc.Batch(enc, images)
+    assert b.info()["items"] == 0
+    poison(b)
+    b.launch()
+    torch.cuda.synchronize()
+    for o, w in zip(b.outputs, want):
+        np.testing.assert_array_equal(o.cpu().numpy(), w)
+    b.close()
+
+
 def test_batch_rejects_bad_images(enc):
     t = torch.zeros((8, 8, 4), dtype=torch.uint8, device="cuda")
     with pytest.raises(astc.AstcError):
         astc.Batch(enc, [(t, 5, 0)])
-    with pytest.raises(astc.AstcError, match="max-accumulation"):
-        astc.Batch(enc, [(t, 4, astc.FLAG_MAX_ACCUMULATION)])

@@ -124,6 +124,16 @@ def test_admission_waits_for_ring_space(enc, pool):
     srv.close()
 
 
+def test_max_accumulation_request_is_rejected(enc):
+    t = torch.zeros((16, 16, 4), dtype=torch.uint8, device="cuda")
+    pk = astc.PackedImages([t], 4, astc.FLAG_ALPHA | astc.FLAG_MAX_ACCUMULATION)
+    srv = astc.Server(enc, 1 << 20, max_inflight=1, max_batch_requests=1)
+    with pytest.raises(astc.AstcError, match="max-accumulation") as e:
+        srv.run_trace([(pk, 0.0, torch.zeros(pk.out_bytes, dtype=torch.uint8, device="cuda"))])
+    assert e.value.code == -4
+    srv.close()
+
+
 def test_oversized_request_is_rejected(enc, pool):
     pk, _ = pool[0]
     srv = astc.Server(enc, 1024, max_inflight=1, max_batch_requests=1)
