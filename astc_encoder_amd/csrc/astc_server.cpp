@@ -102,7 +102,12 @@ struct Slot {
 
 struct ReqGeom {
     std::vector<BatchLevel> levels;  // pointers relative to the request's ring base
-    std::vector<int> modes;
+    std::vector<int> modes;          // -1 (no work-list mode) for plain-launched images
+    // images with the max-accumulation search, encoded by one plain launch each
+    // after the batch's graph: index into levels, description, geometry
+    std::vector<int> plain;
+    std::vector<astc_image_desc> plain_desc;
+    std::vector<ImageGeom> plain_geom;
     int64_t src_bytes = 0, out_bytes = 0, texels = 0, blocks = 0;
 };
 
@@ -184,8 +189,6 @@ int geometry(const astc_request& r, ReqGeom* g) {
         ImageGeom ig;
         int rc = check_desc(&r.images[i], &ig);
         if (rc != ASTC_OK) return rc;
-        if (r.images[i].flags & ASTC_FLAG_MAX_ACCUMULATION)
-            return fail(ASTC_ERR_UNSUPPORTED, "the max-accumulation search is not available in serving (astc_encode and astc_batch have it)");
         if (r.images[i].row_pitch != r.images[i].width * ig.bpp)
             return fail(ASTC_ERR_INVALID_ARG, "request images must be tightly packed");
         BatchLevel& L = g->levels[i];
@@ -201,6 +204,12 @@ int geometry(const astc_request& r, ReqGeom* g) {
         L.dst = reinterpret_cast<uint32_t*>(oo);
         L.nblocks = (int)ig.nblocks;
         g->modes[i] = mode_of(r.images[i].block_dim, r.images[i].src_format, r.images[i].flags, r.images[i].width, r.images[i].height);
+        if (r.images[i].flags & ASTC_FLAG_MAX_ACCUMULATION) {
+            g->modes[i] = -1;  // the planner visits codes 0..MODE_COUNT-1 only
+            g->plain.push_back(i);
+            g->plain_desc.push_back(r.images[i]);
+            g->plain_geom.push_back(ig);
+        }
         so += align_up((int64_t)r.images[i].row_pitch * r.images[i].height);
         oo += align_up(ig.nblocks * 16);
         g->texels += (int64_t)r.images[i].width * r.images[i].height;
@@ -437,6 +446,17 @@ int astc_server_run_trace(astc_server* sv, const astc_request* reqs, int n, doub
                 if (e == hipSuccess)
                     e = hipMemcpyAsync(s.d_scalars, s.h_scalars, 4 * sizeof(int), hipMemcpyHostToDevice, s.stream);
                 if (e == hipSuccess) e = hipGraphLaunch(s.exec, s.stream);
+                for (size_t k = 0; k < s.reqs.size() && e == hipSuccess; ++k) {
+                    const ReqGeom& q = geo[s.reqs[k]];
+                    uint8_t* base = sv->ring_base + s.offs[k];
+                    for (size_t j = 0; j < q.plain.size() && e == hipSuccess; ++j) {
+                        const BatchLevel& L = q.levels[q.plain[j]];
+                        e = launch_encode_plain(sv->ctx->d_tables, q.plain_desc[j], q.plain_geom[j],
+                                                base + reinterpret_cast<uintptr_t>(L.f.src),
+                                                base + q.src_bytes + reinterpret_cast<uintptr_t>(L.dst), s.stream,
+                                                sv->num_cus);
+                    }
+                }
                 for (size_t k = 0; k < s.reqs.size() && e == hipSuccess; ++k) {
                     const int r = s.reqs[k];
                     if (reqs[r].dst_dev)

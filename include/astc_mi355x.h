@@ -54,9 +54,9 @@ extern "C" {
  * 170-227), which the reference leaves commented out in favour of the principal
  * component analysis (:514).  Per channel, the sum of the texel differences from
  * the mean over the texels whose channel is above the mean; the longest sum is
- * the axis.  astc_encode / astc_encode_each, and astc_batch_* (such images run as
- * plain launches inside the batch's graph, after the persistent kernels); the
- * serving loop rejects it with ASTC_ERR_UNSUPPORTED. */
+ * the axis.  Every entry point accepts it; astc_batch_* and astc_server_* encode
+ * such images with one plain launch each, after the persistent kernels (in the
+ * batch's graph, or on the serving batch's stream). */
 #define ASTC_FLAG_MAX_ACCUMULATION 8u
 
 typedef struct astc_image_desc {

@@ -124,13 +124,32 @@ def test_admission_waits_for_ring_space(enc, pool):
     srv.close()
 
 
-def test_max_accumulation_request_is_rejected(enc):
-    t = torch.zeros((16, 16, 4), dtype=torch.uint8, device="cuda")
-    pk = astc.PackedImages([t], 4, astc.FLAG_ALPHA | astc.FLAG_MAX_ACCUMULATION)
-    srv = astc.Server(enc, 1 << 20, max_inflight=1, max_batch_requests=1)
-    with pytest.raises(astc.AstcError, match="max-accumulation") as e:
-        srv.run_trace([(pk, 0.0, torch.zeros(pk.out_bytes, dtype=torch.uint8, device="cuda"))])
-    assert e.value.code == -4
+@pytest.mark.parametrize("tuning", [(4, 2, 1, 0), (8, 1, 0, 1)])
+def test_max_accumulation_requests_in_the_mix(enc, tuning):
+    """Requests with the max-accumulation search (plain launches on the serving
+    batch's stream, in the request's ring space) among ordinary ones, under a
+    small ring: every request equals its plain encodes."""
+    rng = np.random.default_rng(60 + tuning[0])
+    kinds = [(4, 4, astc.FLAG_ALPHA), (6, 4, astc.FLAG_ALPHA | astc.FLAG_SRGB), (4, 3, 0), (4, 2, astc.FLAG_NORMAL)]
+    reqs, wants = [], []
+    for k in range(32):
+        dim, c, flags = kinds[k % len(kinds)]
+        if k % 3 != 2:
+            flags |= astc.FLAG_MAX_ACCUMULATION
+        h, w = int(rng.integers(3, 120)), int(rng.integers(3, 120))
+        img = torch.from_numpy(rng.integers(0, 256, size=(h, w, c), dtype=np.uint8)).cuda()
+        levels = enc.mip_chain(img)
+        pk = astc.PackedImages(levels, dim, flags)
+        wants.append([enc.encode(l, block_dim=dim, flags=flags) for l in levels])
+        reqs.append((pk, float(k) * 2e-5, torch.full((pk.out_bytes,), 0xA5, dtype=torch.uint8, device="cuda")))
+    torch.cuda.synchronize()
+    budget = 3 * max(p.buffer.numel() + p.out_bytes for p, _, _ in reqs)
+    srv = astc.Server(enc, budget, max_inflight=2, max_batch_requests=8, tuning=tuning)
+    lat, st = srv.run_trace(reqs)
+    torch.cuda.synchronize()
+    assert st["requests"] == len(reqs) and st["peak_ring_bytes"] <= budget
+    for (pk, _, dst), want in zip(reqs, wants):
+        check_request(pk, want, dst)
     srv.close()
 
 
