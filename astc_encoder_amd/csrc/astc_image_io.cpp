@@ -185,6 +185,7 @@ int load_png(const char* path, std::vector<uint8_t>* rgba, int* wo, int* ho) {
         zs.next_out = buf;
         zs.avail_out = sizeof(buf);
         zr = inflate(&zs, Z_NO_FLUSH);
+        if (zr == Z_BUF_ERROR) break;  // no progress possible: the input is used up
         if (zr != Z_OK && zr != Z_STREAM_END) {
             inflateEnd(&zs);
             return fail(ASTC_ERR_IO, "corrupt PNG image data");
@@ -194,8 +195,13 @@ int load_png(const char* path, std::vector<uint8_t>* rgba, int* wo, int* ho) {
             inflateEnd(&zs);
             return fail(ASTC_ERR_IO, "PNG image data larger than its header allows");
         }
-    } while (zr != Z_STREAM_END && zs.avail_in > 0);
+        // a full output buffer may leave decoded bytes pending even with no input left
+    } while (zr != Z_STREAM_END && (zs.avail_in > 0 || zs.avail_out == 0));
     inflateEnd(&zs);
+    // Every scanline the header implies must be there before the image is
+    // allocated: the header alone must not size an allocation (a few bytes can
+    // claim 2^24 x 2^24 texels).
+    if (raw.size() < expected) return fail(ASTC_ERR_IO, "PNG image data shorter than its header requires");
 
     rgba->assign(size_t(p.w) * p.h * 4, 0);
     size_t rp = 0;

@@ -128,3 +128,74 @@ def test_cli_without_gpu_paths(tmp_path):
     r = subprocess.run([astc._capi.CLI_PATH, str(tmp_path / "none.png"), "-alpha"], capture_output=True, text=True)
     assert r.returncode == 255 and "load source texture failed!" in r.stdout
     assert "has_alpha\ttrue" in r.stdout
+
+
+def test_header_alone_does_not_size_the_image(tmp_path):
+    """A header claiming 2^24-1 x 2^24-1 RGBA texels over a few bytes of image
+    data is refused for missing data before anything of that size is allocated,
+    as is one scanline short of a small image."""
+    def chunk(tag, data):
+        return struct.pack(">I", len(data)) + tag + data + struct.pack(">I", zlib.crc32(tag + data) & 0xFFFFFFFF)
+    def png(w, h, raw):
+        return (b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 6, 0, 0, 0))
+                + chunk(b"IDAT", zlib.compress(raw, 9)) + chunk(b"IEND", b""))
+    p = tmp_path / "huge.png"
+    p.write_bytes(png((1 << 24) - 1, (1 << 24) - 1, bytes(1 + 4 * 16)))
+    with pytest.raises(astc.AstcError, match="shorter than its header"):
+        astc.load_tex(p)
+    p.write_bytes(png(5, 4, bytes(3 * (1 + 4 * 5))))
+    with pytest.raises(astc.AstcError, match="shorter than its header"):
+        astc.load_tex(p)
+    p.write_bytes(png(5, 4, bytes(4 * (1 + 4 * 5))))
+    assert astc.load_tex(p, flip=False).shape == (4, 5, 4)
+
+
+def test_structured_random_files_decode_or_fail_cleanly(tmp_path):
+    """Seeded random PNGs: header fields (valid and invalid colour type / depth /
+    interlace pairs, odd sizes), scanline data within a few bytes of what the
+    header implies, random filter bytes, PLTE and tRNS of any length, split or
+    missing chunks.  Each either decodes to height x width x 4 or raises
+    AstcError; nothing else (the loader was also run this way under address and
+    undefined-behaviour sanitizers)."""
+    def chunk(tag, data):
+        return struct.pack(">I", len(data)) + tag + data + struct.pack(">I", zlib.crc32(tag + data) & 0xFFFFFFFF)
+    combos = [(0, 1), (0, 2), (0, 4), (0, 8), (0, 16), (2, 8), (2, 16), (3, 1), (3, 2), (3, 4), (3, 8),
+              (4, 8), (4, 16), (6, 8), (6, 16)]
+    rng = np.random.default_rng(77)
+    p = tmp_path / "r.png"
+    decoded = 0
+    for _ in range(400):
+        w, h = int(rng.integers(1, 24)), int(rng.integers(1, 24))
+        ct, dp = combos[rng.integers(len(combos))] if rng.random() < 0.9 else (int(rng.integers(8)), int(rng.integers(20)))
+        il = int(rng.integers(2)) if rng.random() < 0.9 else int(rng.integers(4))
+        bits = {0: 1, 2: 3, 3: 1, 4: 2, 6: 4}.get(ct, 4) * max(dp, 1)
+        passes = [(0, 0, 8, 8), (4, 0, 8, 8), (0, 4, 4, 8), (2, 0, 4, 4), (0, 2, 2, 4), (1, 0, 2, 2), (0, 1, 1, 2)] \
+            if il == 1 else [(0, 0, 1, 1)]
+        raw = bytearray()
+        for ox, oy, dx, dy in passes:
+            pw, ph = max(0, (w - ox + dx - 1) // dx), max(0, (h - oy + dy - 1) // dy)
+            if pw and ph:
+                for _ in range(ph):
+                    row = rng.integers(0, 256, 1 + (pw * bits + 7) // 8, dtype=np.uint8)
+                    row[0] = rng.integers(5) if rng.random() < 0.97 else row[0]
+                    raw += row.tobytes()
+        if rng.random() < 0.2:
+            raw = raw[:max(0, len(raw) + int(rng.integers(-4, 5)))] + bytes(max(0, int(rng.integers(-4, 5))))
+        z = zlib.compress(bytes(raw), int(rng.integers(0, 10)))
+        png = b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, dp, ct, 0, 0, il))
+        if rng.random() < 0.5:
+            png += chunk(b"PLTE", rng.integers(0, 256, int(rng.integers(0, 800)), dtype=np.uint8).tobytes())
+        if rng.random() < 0.5:
+            png += chunk(b"tRNS", rng.integers(0, 256, int(rng.integers(0, 300)), dtype=np.uint8).tobytes())
+        cut = int(rng.integers(0, len(z) + 1))
+        png += chunk(b"IDAT", z[:cut]) + chunk(b"IDAT", z[cut:])
+        if rng.random() < 0.9:
+            png += chunk(b"IEND", b"")
+        p.write_bytes(png)
+        try:
+            img = astc.load_tex(p, flip=bool(rng.integers(2)))
+        except astc.AstcError:
+            continue
+        assert img.shape == (h, w, 4) and img.dtype == np.uint8
+        decoded += 1
+    assert decoded > 100
