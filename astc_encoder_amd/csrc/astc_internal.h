@@ -4,7 +4,6 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -18,7 +17,8 @@ struct astc_ctx {
     astc::DeviceTables h_tables{};           // host copy
     astc::TritQuintDecodeTable* d_dec_tables = nullptr;  // decoder tables
     int num_cus = 256;                       // compute units of the device
-    std::mutex mu;                           // guards lazily created objects
+    // Written only by astc_ctx_create / astc_ctx_destroy: every other call only
+    // reads it, so threads may share one context.
 };
 
 namespace astc {

@@ -13,6 +13,11 @@
  *     throws; astc_last_error() gives a message for the calling thread.
  *   - `stream` is a hipStream_t passed as void* (NULL = the default stream).
  *     Calls are asynchronous with respect to the host unless stated otherwise.
+ *   - Threads: a context is read-only after astc_ctx_create, so any number of
+ *     threads may encode and decode through one context at once.  A batch or a
+ *     server is used by one thread at a time (its calls change its plan, graph
+ *     and ring).  Destroy a context only after every call using it has
+ *     returned, and after its batches and servers are destroyed.
  *   - Output blocks are 16 bytes each in row-major block order
  *     (block (bx, by) at index by * ceil(W/D) + bx), exactly the contents of the
  *     reference's output UAV (RWStructuredBuffer<uint4>, ASTC_Encode.hlsl:81).

@@ -223,3 +223,43 @@ def test_invalid_arguments_raise_before_launch(enc):
     d = astc.make_desc(8, 8, astc.FMT_RGBA8, 4, 0)
     with pytest.raises(astc.AstcError):  # misaligned destination
         enc.encode_raw(d, img.data_ptr(), img.data_ptr() + 4)
+
+
+def test_threads_share_one_context(enc):
+    """Eight host threads encode (and decode) through one context at once, which
+    the context allows (it is read-only after creation): every block equals the
+    one-thread encode of the same image."""
+    import threading
+    rng = np.random.default_rng(8)
+    jobs = []
+    for k in range(48):
+        c = [4, 3, 2][k % 3]
+        h, w = int(rng.integers(1, 200)), int(rng.integers(1, 200))
+        t = torch.from_numpy(rng.integers(0, 256, size=(h, w, c), dtype=np.uint8)).cuda()
+        jobs.append((t, [4, 6][k % 2], int(rng.integers(0, 16))))
+    want = [enc.encode(t, block_dim=d, flags=f) for t, d, f in jobs]
+    want_dec = [enc.decode(o, t.shape[1], t.shape[0], d) for o, (t, d, _) in zip(want, jobs)]
+    outs = [torch.full_like(o, 0xA5) for o in want]
+    decs = [None] * len(jobs)
+    torch.cuda.synchronize()
+    errors = []
+
+    def work(i):
+        try:
+            for k in range(i, len(jobs), 8):
+                t, d, f = jobs[k]
+                enc.encode(t, block_dim=d, flags=f, out=outs[k])
+                decs[k] = enc.decode(outs[k], t.shape[1], t.shape[0], d)
+        except Exception as e:  # surfaced below
+            errors.append(e)
+
+    threads = [threading.Thread(target=work, args=(i,)) for i in range(8)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    torch.cuda.synchronize()
+    assert not errors, errors
+    for k in range(len(jobs)):
+        assert torch.equal(outs[k], want[k]), f"encode {k}"
+        assert torch.equal(decs[k], want_dec[k]), f"decode {k}"
