@@ -258,3 +258,20 @@ def test_batch_rejects_bad_images(enc):
     t = torch.zeros((8, 8, 4), dtype=torch.uint8, device="cuda")
     with pytest.raises(astc.AstcError):
         astc.Batch(enc, [(t, 5, 0)])
+    # output blocks must start on a 16-byte boundary
+    raw = torch.zeros(4 * 16 + 1, dtype=torch.uint8, device="cuda")
+    with pytest.raises(astc.AstcError, match="bad src/dst pointer"):
+        astc.Batch(enc, [(t, 4, 0)], outputs=[raw[1:].view(4, 16)])
+
+
+@pytest.mark.parametrize("tuning", [(3, 2, 1, 0), (4, 0, 1, 0), (4, 5, 1, 0), (4, 2, 2, 0), (4, 2, 1, 2)])
+def test_tuning_out_of_range_is_rejected_and_kept(enc, tuning):
+    t = torch.zeros((8, 8, 4), dtype=torch.uint8, device="cuda")
+    b = astc.Batch(enc, [(t, 4, 0)])
+    before = b.tuning()
+    with pytest.raises(astc.AstcError, match="tuning out of range"):
+        b.set_tuning(*tuning)
+    assert b.tuning() == before
+    b.launch()
+    torch.cuda.synchronize()
+    b.close()

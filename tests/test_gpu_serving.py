@@ -160,3 +160,34 @@ def test_oversized_request_is_rejected(enc, pool):
         srv.run_trace([(pk, 0.0, None)])
     assert e.value.code == -6
     srv.close()
+
+
+@pytest.mark.parametrize("kw", [dict(budget=100), dict(max_inflight=0), dict(max_inflight=9),
+                                dict(max_batch_requests=0), dict(tuning=(3, 2, 1, 0)), dict(tuning=(4, 0, 1, 0)),
+                                dict(tuning=(4, 5, 1, 0)), dict(tuning=(4, 2, 2, 0)), dict(tuning=(4, 2, 1, 2))])
+def test_bad_server_configs_are_rejected(enc, kw):
+    args = dict(budget=1 << 20, max_inflight=2, max_batch_requests=4, tuning=(4, 2, 1, 0))
+    args.update(kw)
+    with pytest.raises(astc.AstcError) as e:
+        astc.Server(enc, args["budget"], max_inflight=args["max_inflight"],
+                    max_batch_requests=args["max_batch_requests"], tuning=args["tuning"])
+    assert e.value.code == -1
+
+
+def test_bad_traces_are_rejected_before_any_work(enc, pool):
+    """Arrival times out of order, and a request whose rows are not tightly packed:
+    refused with nothing written to any destination."""
+    (pk, _), (pk2, _) = pool[0], pool[1]
+    srv = astc.Server(enc, 256 << 20, max_inflight=2, max_batch_requests=4)
+    d1 = torch.full((pk.out_bytes,), 0xA5, dtype=torch.uint8, device="cuda")
+    d2 = torch.full((pk2.out_bytes,), 0xA5, dtype=torch.uint8, device="cuda")
+    with pytest.raises(astc.AstcError, match="arrival") as e:
+        srv.run_trace([(pk, 1e-3, d1), (pk2, 0.0, d2)])
+    assert e.value.code == -1
+    padded = astc.PackedImages([torch.zeros((8, 8, 4), dtype=torch.uint8, device="cuda")], 4, 0)
+    padded.descs[0].row_pitch += 4
+    with pytest.raises(astc.AstcError, match="tightly packed"):
+        srv.run_trace([(pk, 0.0, d1), (padded, 0.0, None)])
+    torch.cuda.synchronize()
+    assert bool((d1 == 0xA5).all()) and bool((d2 == 0xA5).all())
+    srv.close()
