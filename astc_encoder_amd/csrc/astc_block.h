@@ -472,9 +472,13 @@ __device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], co
         // cannot succeed: |M v|^2 = dot(v, S v) with S = M M is the Rayleigh
         // quotient of the positive semi-definite S at the unit v, and it never
         // decreases along power iterates.  The remaining seven iterations use S
-        // (formed once, S_ij = dot(row i, row j)), rescaling between them by a
-        // power of two (exact; only the direction feeds the next product), and
-        // the last one normalises.
+        // (formed once, S_ij = dot(row i, row j)) and the last one normalises.
+        // Only the direction feeds the next product, so the iterate is rescaled
+        // by a power of two (exact) before products 1, 4 and 7: three products
+        // from a largest component in [0.5, 1) stay in the normal range (each
+        // grows it by at most 4 max|S_ij| < 8e10 and shrinks it by at most half
+        // the first test's 1e-10 bound on the Rayleigh quotient), and scaling by
+        // powers of two commutes with rounding, so fewer rescales change no bit.
         float s2[4][4];
 #pragma unroll
         for (int i = 0; i < NC; ++i)
@@ -492,12 +496,17 @@ __device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], co
             for (int i = 0; i < NC; ++i) w[i] = dotn<NC>(m[i], u);
 #pragma unroll
             for (int it = 1; it < 8; ++it) {
-                float mx = fabsf(w[0]);
+                if (it % 3 == 1) {
+                    float mx = fabsf(w[0]);
 #pragma unroll
-                for (int i = 1; i < NC; ++i) mx = fmaxf(mx, fabsf(w[i]));
-                const int e = exponent_of(mx);
+                    for (int i = 1; i < NC; ++i) mx = fmaxf(mx, fabsf(w[i]));
+                    const int e = exponent_of(mx);
 #pragma unroll
-                for (int i = 0; i < NC; ++i) v[i] = ldexp_(w[i], -e);
+                    for (int i = 0; i < NC; ++i) v[i] = ldexp_(w[i], -e);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < NC; ++i) v[i] = w[i];
+                }
 #pragma unroll
                 for (int i = 0; i < NC; ++i) w[i] = dotn<NC>(s2[i], v);
             }

@@ -416,8 +416,12 @@ template <typename T> void matvec(const Arith<T>& ar, const T m[4][4], const T* 
  * the test cannot succeed: with S = M M (S_ij = dot(row i, row j)), |M v|^2 =
  * dot(v, S v) is the Rayleigh quotient of the positive semi-definite S at the
  * unit v, which never decreases along power iterates.  The other seven
- * iterations multiply by S, rescaling by powers of two in between (exact; only
- * the direction feeds the next product); the eighth product is normalised. */
+ * iterations multiply by S; the eighth product is normalised.  The iterate is
+ * rescaled by a power of two (exact; only the direction feeds the next product)
+ * before products 1, 4 and 7: three products from a largest component in
+ * [0.5, 1) stay in the normal range (each grows it by at most 4 max|S_ij| < 8e10
+ * and shrinks it by at most half the first test's 1e-10 bound on the Rayleigh
+ * quotient), and scaling by powers of two commutes with rounding. */
 template <typename T> int exponent_of(T x) {
     int e;
     std::frexp(x, &e);
@@ -435,9 +439,13 @@ template <typename T> void eigen_canonical(const Arith<T>& ar, const T m[4][4], 
     }
     matvec(ar, m, u, w);
     for (int it = 1; it < 8; ++it) {
-        const T mx = std::fmax(std::fmax(std::fabs(w[0]), std::fabs(w[1])), std::fmax(std::fabs(w[2]), std::fabs(w[3])));
-        const int e = exponent_of(mx);
-        for (int i = 0; i < 4; ++i) v[i] = std::ldexp(w[i], -e);
+        if (it % 3 == 1) {
+            const T mx = std::fmax(std::fmax(std::fabs(w[0]), std::fabs(w[1])), std::fmax(std::fabs(w[2]), std::fabs(w[3])));
+            const int e = exponent_of(mx);
+            for (int i = 0; i < 4; ++i) v[i] = std::ldexp(w[i], -e);
+        } else {
+            std::memcpy(v, w, sizeof(w));
+        }
         matvec(ar, s2, v, w);
     }
     std::memcpy(v, w, sizeof(w));
