@@ -381,7 +381,9 @@ __device__ __forceinline__ Block128 pack_block(const uint32_t e0q[4], const uint
 // commented out (:170-227, :514).
 enum : int { METHOD_PCA = 0, METHOD_MAX_ACCUMULATION = 1 };
 
-template <int DIM, bool ALPHA, int NC = 4, int METHOD = METHOD_PCA>
+// RESCALE: the power iteration's rescale interval, 3 or 1 (every product); both
+// give the same blocks (see the power iteration below).
+template <int DIM, bool ALPHA, int NC = 4, int METHOD = METHOD_PCA, int RESCALE = 3>
 __device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], const PackTables* pack) {
     static_assert(NC >= 2 && NC <= 4, "two to four active channels");
     constexpr int N = DIM * DIM;
@@ -474,7 +476,8 @@ __device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], co
         // decreases along power iterates.  The remaining seven iterations use S
         // (formed once, S_ij = dot(row i, row j)) and the last one normalises.
         // Only the direction feeds the next product, so the iterate is rescaled
-        // by a power of two (exact) before products 1, 4 and 7: three products
+        // by a power of two (exact) before products 1, 4 and 7 (RESCALE = 3; 1
+        // rescales before every product, for register budgets that need it): three products
         // from a largest component in [0.5, 1) stay in the normal range (each
         // grows it by at most 4 max|S_ij| < 8e10 and shrinks it by at most half
         // the first test's 1e-10 bound on the Rayleigh quotient), and scaling by
@@ -496,7 +499,7 @@ __device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], co
             for (int i = 0; i < NC; ++i) w[i] = dotn<NC>(m[i], u);
 #pragma unroll
             for (int it = 1; it < 8; ++it) {
-                if (it % 3 == 1) {
+                if ((it - 1) % RESCALE == 0) {
                     float mx = fabsf(w[0]);
 #pragma unroll
                     for (int i = 1; i < NC; ++i) mx = fmaxf(mx, fabsf(w[i]));

@@ -68,6 +68,16 @@ __device__ __forceinline__ void store_block(uint32_t* dst, int id, const Block12
     reinterpret_cast<uint4*>(dst)[id] = make_uint4(b.w[0], b.w[1], b.w[2], b.w[3]);
 }
 
+// Power-iteration rescale interval: the batch kernels' 6x6 path (SRGB < 0: the
+// table is chosen per image) rescales before every product, which keeps the
+// kernels that hold it at 224 VGPRs; every other path rescales before three of
+// the seven (fewer instructions).  Both give the same blocks.  At 224 VGPRs
+// (allocated in steps of 8) a 6x6 wave leaves 288 of a SIMD's 512 for three
+// waves of the 4x4 batch kernel, which run beside it when the split form runs
+// one 6x6 workgroup per CU; one register more leaves room for only two.
+template <int DIM, int SRGB>
+struct BatchRescale { static constexpr int value = (SRGB < 0 && DIM == 6) ? 1 : 3; };
+
 // Encode blocks [first, first + count) of one image with the workgroup's
 // threads: step k, thread t takes block first + k * blockDim.x + t.  `cur`
 // holds the packed texels of the thread's first block (the caller issues those
@@ -86,7 +96,7 @@ __device__ __forceinline__ void encode_loaded(const FetchArgs& f, uint32_t* dst,
         unpack_texels<DIM, SRGB, CH, METHOD == METHOD_MAX_ACCUMULATION>(cur, f, lut, x);
         // the packed words are dead once converted: the next block's loads reuse them
         if (has_next && prefetch) load_block_id<DIM, CH>(f, first + o + stride, cur);
-        store_block(dst, first + o, encode_block<DIM, ALPHA, CH, METHOD>(x, trit));
+        store_block(dst, first + o, encode_block<DIM, ALPHA, CH, METHOD, BatchRescale<DIM, SRGB>::value>(x, trit));
         if (!has_next) break;
         if (!prefetch) load_block_id<DIM, CH>(f, first + o + stride, cur);
     }
@@ -149,7 +159,7 @@ __device__ __forceinline__ void process_item(const BatchArgs& a, const BatchItem
             load_block_id<DIM, CH>(L.f, sg.first_block + o, p);
             float x[DIM * DIM][4];
             unpack_texels<DIM, -1, CH>(p, L.f, lut, x);
-            store_block(L.dst, sg.first_block + o, encode_block<DIM, ALPHA, CH>(x, trit));
+            store_block(L.dst, sg.first_block + o, encode_block<DIM, ALPHA, CH, METHOD_PCA, BatchRescale<DIM, -1>::value>(x, trit));
             return;
         }
         o -= sg.count;

@@ -187,7 +187,7 @@ def batched_vs_plain(enc, steps, warmup, n_textures=512):
     torch.cuda.synchronize()
     gen_s = time.perf_counter() - t0
     b = astc.Batch(enc, images)
-    tuning = b.autotune(reps=2)
+    tuning = b.autotune(reps=10)
     info = b.info()
     stream = torch.cuda.current_stream()
 
