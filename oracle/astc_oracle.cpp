@@ -833,7 +833,10 @@ void fill(uint8_t* rgba, int n, uint8_t r, uint8_t g, uint8_t b, uint8_t a) {
     for (int i = 0; i < n; ++i) { rgba[4 * i] = r; rgba[4 * i + 1] = g; rgba[4 * i + 2] = b; rgba[4 * i + 3] = a; }
 }
 
-int decode_block(const uint8_t blk[16], int bw, int bh, uint8_t* rgba, oracle_block_info* info) {
+/* texel_w (optional): each texel's infilled weight 0..64, plane 0 at [t * bw + s]
+ * and, for dual-plane blocks, plane 1 at [bw * bh + t * bw + s]. */
+int decode_block(const uint8_t blk[16], int bw, int bh, uint8_t* rgba, oracle_block_info* info,
+                 int32_t* texel_w = nullptr) {
     oracle_block_info local;
     oracle_block_info& I = info ? *info : local;
     std::memset(&I, 0, sizeof(I));
@@ -919,6 +922,7 @@ int decode_block(const uint8_t blk[16], int bw, int bh, uint8_t* rgba, oracle_bl
                 if (w10) acc += W(v0 + bm.gw) * w10;
                 if (w11) acc += W(v0 + bm.gw + 1) * w11;
                 wt[p] = (acc + 8) >> 4;
+                if (texel_w) texel_w[p * bw * bh + t * bw + s] = wt[p];
             }
             uint8_t* o = rgba + 4 * (t * bw + s);
             for (int c = 0; c < 4; ++c) {
@@ -1032,6 +1036,19 @@ void oracle_pack_block(const uint32_t ep_q[8], const uint32_t w_q[16], int has_a
 
 int oracle_decode_block(const uint8_t blk[16], int dim_x, int dim_y, uint8_t* rgba, oracle_block_info* info) {
     return decode_block(blk, dim_x, dim_y, rgba, info);
+}
+
+int oracle_decode_block_weights(const uint8_t blk[16], int dim_x, int dim_y, uint8_t* rgba, int32_t* texel_w) {
+    return decode_block(blk, dim_x, dim_y, rgba, nullptr, texel_w);
+}
+
+void oracle_grid6_taps(int32_t idx[64], float wt[64]) {
+    const Taps& tp = taps6();
+    for (int g = 0; g < 16; ++g)
+        for (int k = 0; k < 4; ++k) {
+            idx[4 * g + k] = tp.idx[g][k];
+            wt[4 * g + k] = K<float>::tap[tp.far_count[g][k]];
+        }
 }
 
 int oracle_decode_image(const uint8_t* blocks, int width, int height, int dim, uint8_t* rgba) {

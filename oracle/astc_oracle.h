@@ -108,6 +108,12 @@ typedef struct {
 
 /* Decode one block to RGBA8 texels (dim x dim, row-major, 4 bytes each). */
 int oracle_decode_block(const uint8_t blk[16], int dim_x, int dim_y, uint8_t* rgba, oracle_block_info* info);
+/* As oracle_decode_block, and each texel's infilled weight (0..64, spec C.2.18):
+ * plane 0 at texel_w[t * dim_x + s], plane 1 of a dual-plane block after it. */
+int oracle_decode_block_weights(const uint8_t blk[16], int dim_x, int dim_y, uint8_t* rgba, int32_t* texel_w);
+/* The 6x6 sampling taps the encoder uses (idx_grids / wt_grids, ASTC_Encode.hlsl:268-304):
+ * grid point g's four texel indices idx[4g..4g+3] and tap weights wt[4g..4g+3]. */
+void oracle_grid6_taps(int32_t idx[64], float wt[64]);
 /* Decode a whole image of blocks into a width x height RGBA8 image (row pitch = width*4). */
 int oracle_decode_image(const uint8_t* blocks, int width, int height, int dim, uint8_t* rgba);
 /* Spec ISE decoding of count values. */

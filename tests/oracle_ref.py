@@ -56,6 +56,8 @@ def lib():
             "oracle_assemble_block": (None, [ctypes.c_uint32, ctypes.c_uint32, P, P, P]),
             "oracle_pack_block": (None, [P, P, I, P]),
             "oracle_decode_block": (I, [P, I, I, P, P]),
+            "oracle_decode_block_weights": (I, [P, I, I, P, P]),
+            "oracle_grid6_taps": (None, [P, P]),
             "oracle_decode_image": (I, [P, I, I, I, P]),
             "oracle_ise_decode": (I, [P, I, I, I, P]),
             "oracle_psnr": (ctypes.c_double, [P, I, I, I, I, I, P]),
@@ -114,6 +116,24 @@ def decode_block(blk, dim_x, dim_y=None):
     info = BlockInfo()
     ok = lib().oracle_decode_block(_ptr(b), dim_x, dim_y, _ptr(rgba), ctypes.byref(info))
     return ok, rgba.reshape(dim_y, dim_x, 4), info
+
+
+def decode_block_weights(blk, dim_x, dim_y=None):
+    """Decode one block; returns (ok, rgba, per-texel infilled weights of plane 0 as (dim_y, dim_x))."""
+    dim_y = dim_x if dim_y is None else dim_y
+    b = np.ascontiguousarray(blk, dtype=np.uint8)
+    rgba = np.zeros(dim_x * dim_y * 4, np.uint8)
+    w = np.full(2 * dim_x * dim_y, -1, np.int32)
+    ok = lib().oracle_decode_block_weights(_ptr(b), dim_x, dim_y, _ptr(rgba), _ptr(w))
+    return ok, rgba.reshape(dim_y, dim_x, 4), w[:dim_x * dim_y].reshape(dim_y, dim_x)
+
+
+def grid6_taps():
+    """The oracle's 6x6 sampling taps: (16, 4) texel indices and (16, 4) float32 weights."""
+    idx = np.zeros(64, np.int32)
+    wt = np.zeros(64, np.float32)
+    lib().oracle_grid6_taps(_ptr(idx), _ptr(wt))
+    return idx.reshape(16, 4), wt.reshape(16, 4)
 
 
 def decode_image(blocks, w, h, dim):

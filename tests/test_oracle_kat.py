@@ -217,8 +217,10 @@ def test_canonical_rearrangements_equal_the_shader_formulas_in_binary64(dim, alp
     """The canonical schedule computes the principal axis from S = M M with one
     length test, min/max as dot(texel, v) - dot(mean, v), and weights from texel
     projections.  In binary64 these must give the shader's literal formulas'
-    axis, endpoints and weights (up to rounding): a wrong tap, index or dropped
-    term shows up here as a large difference."""
+    axis, endpoints and weights (up to rounding): a dropped term or a wrong
+    index in one of the rearranged formulas shows up here as a large difference.
+    Both schedules read the same 6x6 tap table, so a wrong tap cannot show up
+    here; test_oracle_grid6.py pins the table against the reference's."""
     rng = np.random.default_rng(21)
     worst_axis = worst_ep = 0.0
     diff_random = 0
