@@ -10,20 +10,28 @@
 namespace astc {
 
 namespace {
-thread_local std::string g_last_error;
+// A fixed buffer per thread: recording an error never allocates, so the error
+// path of an entry point cannot itself fail (for instance after a host
+// allocation has failed).  Longer messages are truncated.
+thread_local char g_last_error[512];
 }
 
-const char* last_error_cstr() { return g_last_error.c_str(); }
+const char* last_error_cstr() { return g_last_error; }
 
-void set_error(const std::string& msg) { g_last_error = msg; }
+void set_error(const char* msg) { std::snprintf(g_last_error, sizeof(g_last_error), "%s", msg ? msg : ""); }
 
-int fail(int code, const std::string& msg) {
+void set_error(const std::string& msg) { set_error(msg.c_str()); }
+
+int fail(int code, const char* msg) {
     set_error(msg);
     return code;
 }
 
+int fail(int code, const std::string& msg) { return fail(code, msg.c_str()); }
+
 int hip_fail(hipError_t e, const char* what) {
-    return fail(ASTC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    std::snprintf(g_last_error, sizeof(g_last_error), "%s: %s", what, hipGetErrorString(e));
+    return ASTC_ERR_HIP;
 }
 
 int check_desc(const astc_image_desc* d, ImageGeom* g) {
@@ -121,6 +129,21 @@ int64_t astc_output_size(const astc_image_desc* desc) {
 }
 
 int astc_encode(astc_ctx* ctx, const astc_image_desc* desc, const void* src_dev, void* dst_dev, void* stream) {
+    return guarded([&]() -> int { return encode_one(ctx, desc, src_dev, dst_dev, stream); });
+}
+
+int astc_decode(astc_ctx* ctx, int width, int height, int block_dim_x, int block_dim_y, const void* blocks_dev,
+                void* rgba_dev, void* stream) {
+    return guarded([&]() -> int {
+        return decode_one(ctx, width, height, block_dim_x, block_dim_y, blocks_dev, rgba_dev, stream);
+    });
+}
+
+}  // extern "C"
+
+namespace astc {
+
+int encode_one(astc_ctx* ctx, const astc_image_desc* desc, const void* src_dev, void* dst_dev, void* stream) {
     if (!ctx) return fail(ASTC_ERR_INVALID_ARG, "ctx is NULL");
     ImageGeom g;
     int rc = check_desc(desc, &g);
@@ -134,8 +157,8 @@ int astc_encode(astc_ctx* ctx, const astc_image_desc* desc, const void* src_dev,
     return ASTC_OK;
 }
 
-int astc_decode(astc_ctx* ctx, int width, int height, int block_dim_x, int block_dim_y, const void* blocks_dev,
-                void* rgba_dev, void* stream) {
+int decode_one(astc_ctx* ctx, int width, int height, int block_dim_x, int block_dim_y, const void* blocks_dev,
+               void* rgba_dev, void* stream) {
     if (!ctx || !blocks_dev || !rgba_dev) return fail(ASTC_ERR_INVALID_ARG, "NULL argument");
     if (width < 1 || height < 1 || width > (1 << 24) - 1 || height > (1 << 24) - 1)
         return fail(ASTC_ERR_INVALID_ARG, "width/height must be in 1..16777215");
@@ -153,12 +176,16 @@ int astc_decode(astc_ctx* ctx, int width, int height, int block_dim_x, int block
     return ASTC_OK;
 }
 
+}  // namespace astc
+
+extern "C" {
+
 int astc_encode_each(astc_ctx* ctx, const astc_image_desc* descs, const void* const* srcs, void* const* dsts,
                      int count, void* stream) {
     return guarded([&]() -> int {
         if (!descs || !srcs || !dsts || count < 0) return fail(ASTC_ERR_INVALID_ARG, "NULL arrays");
         for (int i = 0; i < count; ++i) {
-            int rc = astc_encode(ctx, &descs[i], srcs[i], dsts[i], stream);
+            int rc = encode_one(ctx, &descs[i], srcs[i], dsts[i], stream);
             if (rc != ASTC_OK) return rc;
         }
         return ASTC_OK;

@@ -24,7 +24,11 @@ struct astc_ctx {
 namespace astc {
 
 // thread-local last error
+// (set_error and fail copy the message into a fixed per-thread buffer and never
+// allocate)
+void set_error(const char* msg);
 void set_error(const std::string& msg);
+int fail(int code, const char* msg);
 int fail(int code, const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
 
@@ -37,7 +41,8 @@ struct ImageGeom {
 int check_desc(const astc_image_desc* d, ImageGeom* g);
 
 // Runs an entry point's body so that no C++ exception crosses the C ABI: a
-// failed host allocation becomes ASTC_ERR_NO_MEMORY.
+// failed host allocation becomes ASTC_ERR_NO_MEMORY.  The handlers record their
+// message without allocating.
 template <typename F>
 auto guarded(F&& f) -> decltype(f()) {
     try {
@@ -45,11 +50,17 @@ auto guarded(F&& f) -> decltype(f()) {
     } catch (const std::bad_alloc&) {
         return fail(ASTC_ERR_NO_MEMORY, "host allocation failed");
     } catch (const std::exception& e) {
-        return fail(ASTC_ERR_INVALID_ARG, std::string("unexpected error: ") + e.what());
+        set_error(e.what());
+        return ASTC_ERR_INVALID_ARG;
     } catch (...) {
         return fail(ASTC_ERR_INVALID_ARG, "unexpected error");
     }
 }
+
+// Bodies of astc_encode / astc_decode (the entry points run them under guarded).
+int encode_one(astc_ctx* ctx, const astc_image_desc* desc, const void* src_dev, void* dst_dev, void* stream);
+int decode_one(astc_ctx* ctx, int width, int height, int block_dim_x, int block_dim_y, const void* blocks_dev,
+               void* rgba_dev, void* stream);
 
 // Host construction of the tables (identical on every host: sRGB via double pow,
 // rounded to binary32, times 255.0f; trits from the specification decoder).
