@@ -108,7 +108,11 @@ struct ReqGeom {
     std::vector<int> plain;
     std::vector<astc_image_desc> plain_desc;
     std::vector<ImageGeom> plain_geom;
-    int64_t src_bytes = 0, out_bytes = 0, texels = 0, blocks = 0;
+    // src_copy: bytes the caller's source buffer holds (the last image ends
+    // there); out_bytes: the blocks, image after image with no gaps (16-byte
+    // aligned since every image's blocks are); src_bytes: the ring's room for
+    // the sources (src_copy rounded up, so the blocks start 256-byte aligned)
+    int64_t src_copy = 0, src_bytes = 0, out_bytes = 0, texels = 0, blocks = 0;
 };
 
 }  // namespace
@@ -210,12 +214,14 @@ int geometry(const astc_request& r, ReqGeom* g) {
             g->plain_desc.push_back(r.images[i]);
             g->plain_geom.push_back(ig);
         }
-        so += align_up((int64_t)r.images[i].row_pitch * r.images[i].height);
-        oo += align_up(ig.nblocks * 16);
+        const int64_t bytes = (int64_t)r.images[i].row_pitch * r.images[i].height;
+        g->src_copy = so + bytes;
+        so += align_up(bytes);
+        oo += ig.nblocks * 16;
         g->texels += (int64_t)r.images[i].width * r.images[i].height;
         g->blocks += ig.nblocks;
     }
-    g->src_bytes = so;
+    g->src_bytes = align_up(g->src_copy);
     g->out_bytes = oo;
     return ASTC_OK;
 }
@@ -233,7 +239,7 @@ int64_t astc_request_bytes(const astc_image_desc* images, int nimages) {
         ReqGeom g;
         int rc = geometry(r, &g);
         if (rc != ASTC_OK) return rc;
-        return g.src_bytes + g.out_bytes;
+        return g.src_bytes + align_up(g.out_bytes);
     });
 }
 
@@ -339,7 +345,7 @@ int astc_server_run_trace(astc_server* sv, const astc_request* reqs, int n, doub
             if (rc != ASTC_OK) return fail(rc, "request " + std::to_string(i) + ": " + astc_last_error());
             if (i && reqs[i].arrival_s < reqs[i - 1].arrival_s)
                 return fail(ASTC_ERR_INVALID_ARG, "arrival times must be non-decreasing");
-            if (geo[i].src_bytes + geo[i].out_bytes > sv->cfg.hbm_budget_bytes)
+            if (geo[i].src_bytes + align_up(geo[i].out_bytes) > sv->cfg.hbm_budget_bytes)
                 return fail(ASTC_ERR_BUDGET, "request " + std::to_string(i) + " is larger than the HBM budget");
             st.images += reqs[i].nimages;
             st.texels += geo[i].texels;
@@ -396,7 +402,7 @@ int astc_server_run_trace(astc_server* sv, const astc_request* reqs, int n, doub
                 while (!waiting.empty() && (int)s.reqs.size() < sv->cfg.max_batch_requests) {
                     const int r = waiting.front();
                     int64_t off;
-                    if (!ring.alloc(geo[r].src_bytes + geo[r].out_bytes, &off)) break;
+                    if (!ring.alloc(geo[r].src_bytes + align_up(geo[r].out_bytes), &off)) break;
                     waiting.pop_front();
                     s.reqs.push_back(r);
                     s.offs.push_back(off);
@@ -431,7 +437,7 @@ int astc_server_run_trace(astc_server* sv, const astc_request* reqs, int n, doub
                 e = hipSuccess;
                 for (size_t k = 0; k < s.reqs.size() && e == hipSuccess; ++k) {
                     const int r = s.reqs[k];
-                    e = hipMemcpyAsync(sv->ring_base + s.offs[k], reqs[r].src_dev, geo[r].src_bytes,
+                    e = hipMemcpyAsync(sv->ring_base + s.offs[k], reqs[r].src_dev, geo[r].src_copy,
                                        hipMemcpyDeviceToDevice, s.stream);
                 }
                 if (e == hipSuccess)

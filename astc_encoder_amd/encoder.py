@@ -237,22 +237,24 @@ def _align256(n):
 
 class PackedImages:
     """A request's images (e.g. one mip chain) packed back to back in one device
-    buffer, image i at offset sum_{j<i} align256(bytes_j): the layout
-    astc_server_run_trace expects.  Also sizes the matching block layout."""
+    buffer, image i at offset sum_{j<i} align256(bytes_j), the buffer ending with
+    the last image's bytes: the layout astc_server_run_trace expects.  Also
+    sizes the matching block layout (image after image, no gaps)."""
 
     def __init__(self, tensors, block_dim, flags):
         import torch
         n = len(tensors)
         self.descs = (ImageDesc * n)()
-        offs, total, out_offs, out_total = [], 0, [], 0
+        offs, total, out_offs, out_total, end = [], 0, [], 0, 0
         for i, t in enumerate(tensors):
             h, w, c = t.shape
             self.descs[i] = make_desc(w, h, _CHANNELS_TO_FMT[c], block_dim, flags)
             offs.append(total)
+            end = total + h * w * c
             total += _align256(h * w * c)
             out_offs.append(out_total)
-            out_total += _align256(output_size(self.descs[i]))
-        self.buffer = torch.empty(max(total, 256), dtype=torch.uint8, device=tensors[0].device)
+            out_total += output_size(self.descs[i])
+        self.buffer = torch.empty(end, dtype=torch.uint8, device=tensors[0].device)
         for t, o in zip(tensors, offs):
             self.buffer[o:o + t.numel()].copy_(t.contiguous().view(-1))
         self.nimages = n

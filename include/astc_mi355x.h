@@ -168,10 +168,13 @@ int astc_batch_launch(astc_batch* batch, void* stream);
 int astc_batch_info(const astc_batch* batch, int64_t* blocks, int64_t* texels, int32_t* items);
 int astc_batch_destroy(astc_batch* batch);
 
+
 /* ---- serving: continuous batching of compression requests ------------------
  * A request is a set of images (typically one texture's mip chain) whose
  * tightly packed sources lie back to back in device memory, image i at offset
- * sum over j < i of align256(width_j * height_j * bytes per texel).  The server
+ * sum over j < i of align256(width_j * height_j * bytes per texel); the buffer
+ * needs to hold only up to the last image's last byte, and exactly that many
+ * bytes are read.  The server
  * owns a device ring of `hbm_budget_bytes`: an admitted request's sources are
  * copied into it (one device-to-device copy) next to space for its blocks; a
  * request waits while the ring lacks room.  Up to `max_inflight` batches run at
@@ -193,7 +196,9 @@ typedef struct astc_request {
     int32_t nimages;
     int32_t reserved;
     const void* src_dev;           /* the images' texels, laid out as described above */
-    void* dst_dev;                 /* NULL, or where the blocks (image after image) are copied on completion */
+    void* dst_dev;                 /* NULL, or where the blocks are copied on completion: image after image
+                                      with no gaps (image i's 16-byte blocks at the sum of the block bytes
+                                      of images j < i), 16-byte aligned, exactly that many bytes written */
     double arrival_s;              /* arrival time, seconds after the start of the trace */
 } astc_request;
 
@@ -207,7 +212,8 @@ typedef struct astc_server_stats {
     double max_loop_gap_s;       /* longest pass of the serving loop (host stalls show here) */
 } astc_server_stats;
 
-/* Bytes the ring holds for one request (sources + blocks, 256-byte aligned). */
+/* Bytes the ring holds for one request: its sources and its blocks, each
+ * rounded up to a multiple of 256. */
 int64_t astc_request_bytes(const astc_image_desc* images, int nimages);
 int astc_server_create(astc_ctx* ctx, const astc_server_config* config, astc_server** out);
 /* Serve `n` requests (arrival_s non-decreasing) in real time on the calling

@@ -191,3 +191,27 @@ def test_bad_traces_are_rejected_before_any_work(enc, pool):
     torch.cuda.synchronize()
     assert bool((d1 == 0xA5).all()) and bool((d2 == 0xA5).all())
     srv.close()
+
+
+def test_exact_size_buffers_and_tight_block_layout(enc, pool):
+    """Sources end with the last image's bytes and blocks land image after image
+    with no gaps, exactly out_bytes of them: a guard zone after each destination
+    keeps its sentinel, and astc_request_bytes is the two parts rounded to 256."""
+    srv = astc.Server(enc, 256 << 20, max_inflight=2, max_batch_requests=8)
+    reqs, guards = [], []
+    for k, (pk, want) in enumerate(pool):
+        last = pk.descs[pk.nimages - 1]
+        assert pk.buffer.numel() == pk.src_offsets[-1] + last.width * last.height * {0: 4, 1: 3, 2: 2}[last.src_format]
+        assert pk.out_bytes == sum(int(w.numel()) for w in want)
+        full = torch.full((pk.out_bytes + 512,), 0xA5, dtype=torch.uint8, device="cuda")
+        reqs.append((pk, k * 1e-5, full[:pk.out_bytes]))
+        guards.append(full)
+        rb = astc.lib().astc_request_bytes(pk.descs, pk.nimages)
+        src_room = (pk.buffer.numel() + 255) // 256 * 256
+        assert rb == src_room + (pk.out_bytes + 255) // 256 * 256
+    srv.run_trace(reqs)
+    torch.cuda.synchronize()
+    for (pk, want), (_, _, dst), full in zip(pool, reqs, guards):
+        np.testing.assert_array_equal(dst.cpu().numpy(), torch.cat([w.view(-1) for w in want]).cpu().numpy())
+        assert bool((full[pk.out_bytes:] == 0xA5).all())
+    srv.close()
