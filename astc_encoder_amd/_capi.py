@@ -98,6 +98,8 @@ SIGNATURES = {
     "astc_batch_launch": (I32, [P, P]),
     "astc_batch_info": (I32, [P, ctypes.POINTER(I64), ctypes.POINTER(I64), ctypes.POINTER(I32)]),
     "astc_batch_destroy": (I32, [P]),
+    "astc_encode_batch_noalloc": (I32, [P, PDESC, ctypes.POINTER(P), ctypes.POINTER(P), I32, ctypes.POINTER(Tuning),
+                                        P]),
     "astc_request_bytes": (I64, [PDESC, I32]),
     "astc_server_create": (I32, [P, ctypes.POINTER(ServerConfig), ctypes.POINTER(P)]),
     "astc_server_run_trace": (I32, [P, ctypes.POINTER(Request), I32, ctypes.POINTER(ctypes.c_double),

@@ -43,63 +43,106 @@ namespace astc {
 // Mode code of an image that is not in the work list (no code the planner visits).
 constexpr int kNoMode = -1;
 
+// Position of each mode code in the planner's visiting order: most expensive
+// first (block size, then active channels, alpha, sRGB), ties by code.
+int mode_position(int mode) {
+    int pos = 0;
+    for (int m = 0; m < MODE_COUNT; ++m)
+        pos += mode_cost_rank(m) > mode_cost_rank(mode) || (mode_cost_rank(m) == mode_cost_rank(mode) && m < mode);
+    return pos;
+}
+
 // Work list for a tuning.  Modes are ordered by cost (6x6 RGBA first) so the
-// longest items start first; within a mode, images by size, large ones split
-// into chunks of 256 x blocks_per_lane blocks, images under 256 blocks packed
-// up to kMaxSegs per item (one block per thread).
+// longest items start first; within a mode, images by size (large first, then
+// by index), large ones split into chunks of 256 x blocks_per_lane blocks,
+// images under 256 blocks packed up to kMaxSegs per item (one block per
+// thread).  The split point (`first_light`) is the first item of the first
+// 4x4 reduced-channel mode in that order (4x4 four-channel items rank above
+// them and go with the 6x6 items).  Codes with both class bits set are not
+// modes (no image has them); images with a mode outside 0..MODE_COUNT-1 are
+// left out.  Writes into the caller's arrays (order: nlevels ints of room) and
+// allocates nothing; returns false if segs or items would exceed their capacity.
+bool plan_work_into(const BatchLevel* levels, const int* modes, int nlevels, int blocks_per_lane, int* order,
+                    BatchSeg* segs, int seg_cap, BatchItem* items, int item_cap, int* nseg, int* nitem,
+                    int* first_light) {
+    int pos[MODE_COUNT];
+    int light_pos = MODE_COUNT;
+    for (int m = 0; m < MODE_COUNT; ++m) {
+        pos[m] = mode_position(m);
+        const int ch = m & MODE_CH_MASK;
+        if (!(m & 2) && (ch == MODE_CH_RGB8 || ch == MODE_CH_NORMAL)) light_pos = std::min(light_pos, pos[m]);
+    }
+    int n = 0;
+    for (int i = 0; i < nlevels; ++i)
+        if (modes[i] >= 0 && modes[i] < MODE_COUNT) order[n++] = i;
+    // a total order (index breaks ties), so the in-place sort is deterministic
+    std::sort(order, order + n, [&](int a, int b) {
+        if (pos[modes[a]] != pos[modes[b]]) return pos[modes[a]] < pos[modes[b]];
+        if (levels[a].nblocks != levels[b].nblocks) return levels[a].nblocks > levels[b].nblocks;
+        return a < b;
+    });
+    const int chunk = kPlainThreads * blocks_per_lane;
+    int ns = 0, ni = 0;
+    *first_light = -1;
+    BatchItem pack{0, 0, 0, 0};
+    int pack_blocks = 0, cur_mode = -1;
+    auto flush = [&]() -> bool {
+        if (pack.nseg) {
+            if (ni >= item_cap) return false;
+            items[ni++] = pack;
+        }
+        pack = BatchItem{cur_mode, ns, 0, 0};
+        pack_blocks = 0;
+        return true;
+    };
+    for (int k = 0; k < n; ++k) {
+        const int li = order[k], mode = modes[li];
+        if (mode != cur_mode) {
+            if (!flush()) return false;
+            cur_mode = mode;
+            pack = BatchItem{mode, ns, 0, 0};
+        }
+        if (*first_light < 0 && pos[mode] >= light_pos) *first_light = ni;
+        const int nb = levels[li].nblocks;
+        if (nb >= kPlainThreads) {
+            for (int first = 0; first < nb; first += chunk) {
+                if (ni >= item_cap || ns >= seg_cap) return false;
+                items[ni++] = BatchItem{mode, ns, 1, 0};
+                segs[ns++] = BatchSeg{li, first, std::min(chunk, nb - first), 0};
+            }
+            pack.seg_begin = ns;
+            continue;
+        }
+        if (pack_blocks + nb > kPlainThreads || pack.nseg == kMaxSegs) {
+            if (!flush()) return false;  // packs stay contiguous in the segment array
+        }
+        if (ns >= seg_cap) return false;
+        segs[ns++] = BatchSeg{li, 0, nb, 0};
+        ++pack.nseg;
+        pack_blocks += nb;
+    }
+    if (!flush()) return false;
+    if (*first_light < 0) *first_light = ni;
+    *nseg = ns;
+    *nitem = ni;
+    return true;
+}
+
 void plan_work(const std::vector<BatchLevel>& levels, const std::vector<int>& modes, int blocks_per_lane,
                std::vector<BatchSeg>* segs, std::vector<BatchItem>* items, int* first_light) {
-    segs->clear();
-    items->clear();
-    *first_light = -1;
-    const int chunk = kPlainThreads * blocks_per_lane;
-    int order[MODE_COUNT];
-    for (int m = 0; m < MODE_COUNT; ++m) order[m] = m;
-    std::stable_sort(order, order + MODE_COUNT, [](int a, int b) { return mode_cost_rank(a) > mode_cost_rank(b); });
-    for (int mode : order) {
-        // split point: the first 4x4 item of a reduced-channel class (4x4
-        // four-channel items rank above them and go with the 6x6 items).  The
-        // class is named explicitly: codes with both class bits set are not
-        // modes (no image has them) and rank with the four-channel modes.
-        const int ch = mode & MODE_CH_MASK;
-        if (!(mode & 2) && (ch == MODE_CH_RGB8 || ch == MODE_CH_NORMAL) && *first_light < 0)
-            *first_light = (int)items->size();
-        std::vector<int> idx;
-        for (int i = 0; i < (int)levels.size(); ++i)
-            if (modes[i] == mode) idx.push_back(i);
-        std::stable_sort(idx.begin(), idx.end(),
-                         [&](int a, int b) { return levels[a].nblocks > levels[b].nblocks; });
-        BatchItem pack{mode, 0, 0, 0};
-        int pack_blocks = 0;
-        auto flush = [&]() {
-            if (pack.nseg) items->push_back(pack);
-            pack = BatchItem{mode, (int)segs->size(), 0, 0};
-            pack_blocks = 0;
-        };
-        pack.seg_begin = (int)segs->size();
-        for (int li : idx) {
-            const int n = levels[li].nblocks;
-            if (n >= kPlainThreads) {
-                for (int first = 0; first < n; first += chunk) {
-                    items->push_back(BatchItem{mode, (int)segs->size(), 1, 0});
-                    segs->push_back(BatchSeg{li, first, std::min(chunk, n - first), 0});
-                }
-                pack.seg_begin = (int)segs->size();
-                continue;
-            }
-            if (pack_blocks + n > kPlainThreads || pack.nseg == kMaxSegs) {
-                // keep packs contiguous in the segment array
-                if (pack.nseg) items->push_back(pack);
-                pack = BatchItem{mode, (int)segs->size(), 0, 0};
-                pack_blocks = 0;
-            }
-            segs->push_back(BatchSeg{li, 0, n, 0});
-            ++pack.nseg;
-            pack_blocks += n;
-        }
-        flush();
-    }
-    if (*first_light < 0) *first_light = (int)items->size();
+    // every image yields at most ceil(nblocks / chunk) segments, and every item
+    // holds at least one segment
+    const int64_t chunk = int64_t(kPlainThreads) * blocks_per_lane;
+    int64_t cap = 1;
+    for (const BatchLevel& L : levels) cap += (L.nblocks + chunk - 1) / chunk;
+    std::vector<int> order(levels.size());
+    segs->resize(size_t(cap));
+    items->resize(size_t(cap));
+    int ns = 0, ni = 0;
+    plan_work_into(levels.data(), modes.data(), (int)levels.size(), blocks_per_lane, order.data(), segs->data(),
+                   (int)cap, items->data(), (int)cap, &ns, &ni, first_light);
+    segs->resize(size_t(ns));
+    items->resize(size_t(ni));
 }
 
 }  // namespace astc

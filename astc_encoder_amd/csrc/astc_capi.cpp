@@ -108,6 +108,11 @@ int astc_ctx_create(int device, astc_ctx** out) {
         astc_ctx_destroy(ctx);
         return hip_fail(e, "context tables");
     }
+    if (int rc = noalloc_setup(ctx); rc != ASTC_OK) {
+        astc_ctx_destroy(ctx);
+        set_error(std::string("context set-up: ") + astc_last_error());
+        return rc;
+    }
     *out = ctx;
     return ASTC_OK;
 }
@@ -117,6 +122,7 @@ int astc_ctx_destroy(astc_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->d_tables) (void)hipFree(ctx->d_tables);
     if (ctx->d_dec_tables) (void)hipFree(ctx->d_dec_tables);
+    noalloc_teardown(ctx);
     delete ctx;
     return ASTC_OK;
 }

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cstddef>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -11,14 +13,31 @@
 #include "../../include/astc_mi355x.h"
 #include "astc_block.h"
 
+namespace astc {
+struct NoAllocPlan;
+// State of astc_encode_batch_noalloc, set up with the context: the device
+// address of the library's scratch array on the context's device, pinned host
+// staging of the same layout, the event after which the staging may be reused,
+// and host work arrays.  The mutex serialises the entry point's use of them.
+struct NoAllocState {
+    void* scratch = nullptr;
+    NoAllocPlan* staging = nullptr;
+    hipEvent_t staged = nullptr;
+    std::mutex mu;
+    int order[8192];
+    int modes[8192];
+};
+}  // namespace astc
+
 struct astc_ctx {
     int device = 0;
     astc::DeviceTables* d_tables = nullptr;  // device copy of the sRGB / trit tables
     astc::DeviceTables h_tables{};           // host copy
     astc::TritQuintDecodeTable* d_dec_tables = nullptr;  // decoder tables
     int num_cus = 256;                       // compute units of the device
-    // Written only by astc_ctx_create / astc_ctx_destroy: every other call only
-    // reads it, so threads may share one context.
+    astc::NoAllocState noalloc;              // astc_encode_batch_noalloc only (it locks noalloc.mu)
+    // Otherwise written only by astc_ctx_create / astc_ctx_destroy: every other
+    // call only reads it, so threads may share one context.
 };
 
 namespace astc {
@@ -116,6 +135,31 @@ inline int mode_cost_rank(int mode) {
     const int ch = (mode & MODE_CH_MASK) == MODE_CH_NORMAL ? 0 : (mode & MODE_CH_MASK) == MODE_CH_RGB8 ? 1 : 2;
     return ((mode & 2) ? 16 : 0) + ch * 4 + (mode & 1) * 2 + ((mode & MODE_SRGB) ? 1 : 0);
 }
+
+// Work-list planner (astc_batch.cpp) into caller arrays, allocation-free;
+// false if the segments or items exceed their capacity.  plan_work sizes
+// vectors for it.
+bool plan_work_into(const BatchLevel* levels, const int* modes, int nlevels, int blocks_per_lane, int* order,
+                    BatchSeg* segs, int seg_cap, BatchItem* items, int item_cap, int* nseg, int* nitem,
+                    int* first_light);
+
+// astc_encode_batch_noalloc's limits and the layout of its scratch array (and
+// of the pinned host staging it is uploaded from).
+constexpr int kNoAllocMaxImages = 8192;
+constexpr int kNoAllocMaxItems = 131072;
+struct NoAllocPlan {
+    BatchLevel levels[kNoAllocMaxImages];
+    BatchSeg segs[kNoAllocMaxItems];
+    BatchItem items[kNoAllocMaxItems];
+    int scalars[4];  // item count, first 4x4 item, two queue heads
+};
+constexpr size_t kNoAllocScratchBytes = (sizeof(NoAllocPlan) + 255) / 256 * 256;
+static_assert(sizeof(BatchLevel) == 64 && sizeof(BatchSeg) == 16 && sizeof(BatchItem) == 16, "work-list layout");
+static_assert(sizeof(NoAllocState::order) / sizeof(int) == kNoAllocMaxImages, "no-alloc host arrays");
+// Set up / release the no-alloc state of a context (astc_noalloc.hip), with the
+// context's device current.
+int noalloc_setup(astc_ctx* ctx);
+void noalloc_teardown(astc_ctx* ctx);
 
 hipError_t launch_downsample(const uint8_t* src, int w, int h, int bpp, uint8_t* dst, int ow, int oh,
                              hipStream_t stream);

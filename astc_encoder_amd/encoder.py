@@ -145,6 +145,30 @@ class Encoder:
                                     ctypes.c_void_p(out.data_ptr()), _stream_handle(stream)))
         return out
 
+    def encode_batch_noalloc(self, images, outputs, tuning=None, stream=None):
+        """astc_encode_batch_noalloc: encode images [(device tensor (H, W, C)
+        uint8, block_dim, flags), ...] into outputs [(nblocks, 16) uint8 device
+        tensors] with the persistent kernel, allocating nothing (the work list
+        lives in the library's fixed device array).  tuning: None or
+        (blocks_per_lane, workgroups_per_cu, prefetch)."""
+        n = len(images)
+        if len(outputs) != n:
+            raise ValueError("one output per image")
+        descs = (ImageDesc * n)()
+        srcs = (ctypes.c_void_p * n)()
+        dsts = (ctypes.c_void_p * n)()
+        for i, ((tex, dim, flags), out) in enumerate(zip(images, outputs)):
+            h, w, c = tex.shape
+            if tex.stride(2) != 1 or tex.stride(1) != c:
+                raise ValueError("image rows must have contiguous texels")
+            descs[i] = make_desc(w, h, _CHANNELS_TO_FMT[c], dim, flags, row_pitch=tex.stride(0))
+            if out.numel() < output_size(descs[i]) or not out.is_contiguous():
+                raise ValueError(f"output {i} is too small or not contiguous")
+            srcs[i] = tex.data_ptr()
+            dsts[i] = out.data_ptr()
+        t = None if tuning is None else ctypes.byref(Tuning(int(tuning[0]), int(tuning[1]), int(tuning[2]), 0))
+        check(lib().astc_encode_batch_noalloc(self._ctx, descs, srcs, dsts, n, t, _stream_handle(stream)))
+
     def mip_chain(self, base, stream=None):
         """[base, level1, ...] while a side can halve, generated on the device."""
         levels = [base]

@@ -13,8 +13,9 @@
  *     throws; astc_last_error() gives a message for the calling thread.
  *   - `stream` is a hipStream_t passed as void* (NULL = the default stream).
  *     Calls are asynchronous with respect to the host unless stated otherwise.
- *   - Threads: a context is read-only after astc_ctx_create, so any number of
- *     threads may encode and decode through one context at once.  A batch or a
+ *   - Threads: a context is read-only after astc_ctx_create (except for
+ *     astc_encode_batch_noalloc, which serialises its own use of it), so any
+ *     number of threads may encode and decode through one context at once.  A batch or a
  *     server is used by one thread at a time (its calls change its plan, graph
  *     and ring).  Destroy a context only after every call using it has
  *     returned, and after its batches and servers are destroyed.
@@ -168,6 +169,22 @@ int astc_batch_launch(astc_batch* batch, void* stream);
 int astc_batch_info(const astc_batch* batch, int64_t* blocks, int64_t* texels, int32_t* items);
 int astc_batch_destroy(astc_batch* batch);
 
+/* Encode `count` images with the batched persistent kernel in one call that
+ * allocates no device or host memory: the work list and queue heads live in a
+ * fixed array in the library (4.5 MiB per GPU, held until the process ends),
+ * uploaded from pinned staging the context allocated at creation.  No
+ * counterpart in the reference: this is for callers that cannot allocate
+ * (code under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Images and outputs as for astc_batch_create; each image gets
+ * exactly the blocks astc_encode gives it.  Limits: at most 8192 images and
+ * 131072 work items (chunks of 256 x blocks_per_lane blocks; images under 256
+ * blocks share items); larger batches are refused with ASTC_ERR_INVALID_ARG.
+ * tuning: NULL for {4, 2, 1, 0}; split_modes must be 0 (one kernel, one
+ * stream).  Launches on `stream`, stream-ordered; calls on one GPU share the
+ * array, so calls that overlap on the device (different streams) race, and
+ * calls from several threads are serialised on the host. */
+int astc_encode_batch_noalloc(astc_ctx* ctx, const astc_image_desc* descs, const void* const* srcs,
+                              void* const* dsts, int count, const astc_tuning* tuning, void* stream);
 
 /* ---- serving: continuous batching of compression requests ------------------
  * A request is a set of images (typically one texture's mip chain) whose
