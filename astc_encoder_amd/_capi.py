@@ -45,6 +45,16 @@ class Tuning(ctypes.Structure):
                 "prefetch": self.prefetch, "split_modes": self.split_modes}
 
 
+class PlainTuning(ctypes.Structure):
+    """astc_plain_tuning"""
+    _fields_ = [("lanes_per_block", ctypes.c_int32), ("blocks_per_lane", ctypes.c_int32),
+                ("prefetch", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {"lanes_per_block": self.lanes_per_block, "blocks_per_lane": self.blocks_per_lane,
+                "prefetch": self.prefetch}
+
+
 class ServerConfig(ctypes.Structure):
     """astc_server_config"""
     _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("max_inflight", ctypes.c_int32),
@@ -80,6 +90,8 @@ SIGNATURES = {
     "astc_ctx_destroy": (I32, [P]),
     "astc_output_size": (I64, [ctypes.POINTER(ImageDesc)]),
     "astc_encode": (I32, [P, ctypes.POINTER(ImageDesc), P, P, P]),
+    "astc_encode_tuned": (I32, [P, ctypes.POINTER(ImageDesc), P, P, ctypes.POINTER(PlainTuning), P]),
+    "astc_plain_autotune": (I32, [P, ctypes.POINTER(ImageDesc), P, P, P, I32, ctypes.POINTER(PlainTuning)]),
     "astc_decode": (I32, [P, I32, I32, I32, I32, P, P, P]),
     "astc_encode_each": (I32, [P, PDESC, ctypes.POINTER(P), ctypes.POINTER(P), I32, P]),
     "astc_write_file": (I32, [ctypes.c_char_p, I32, I32, I32, I32, P, I64]),

@@ -189,16 +189,16 @@ __device__ __forceinline__ void load_row_expand(const FetchArgs& a, const uint8_
     }
 }
 
-// The texel rows of a fully in-bounds block whose first row starts at `row`,
-// in the class's layout.  The source format is tested once per block.
-template <int DIM, int CH>
+// NR texel rows of a fully in-bounds block, the first starting at `row`, in
+// the class's layout.  The source format is tested once per block.
+template <int DIM, int CH, int NR = DIM>
 __device__ __forceinline__ void load_rows_fast(const FetchArgs& a, const uint8_t* row, uint32_t* out) {
     using L = Layout<DIM, CH>;
     const int pitch = a.row_pitch;
     if (CH == CH_RGB8 && DIM == 4) {
         // 12 bytes at 12 bx past an aligned row start: three aligned words
 #pragma unroll
-        for (int r = 0; r < DIM; ++r, row += pitch)
+        for (int r = 0; r < NR; ++r, row += pitch)
 #pragma unroll
             for (int i = 0; i < L::WPR; ++i) out[r * L::WPR + i] = reinterpret_cast<const uint32_t*>(row)[i];
     } else if (CH == CH_RGB8) {
@@ -206,7 +206,7 @@ __device__ __forceinline__ void load_rows_fast(const FetchArgs& a, const uint8_t
         const uint32_t shift = uint32_t(reinterpret_cast<uintptr_t>(row) & 3u);
         row -= shift;
 #pragma unroll
-        for (int r = 0; r < DIM; ++r, row += pitch) {
+        for (int r = 0; r < NR; ++r, row += pitch) {
             const uint32_t* w = reinterpret_cast<const uint32_t*>(row);
             uint32_t raw[L::WPR + 1];
 #pragma unroll
@@ -219,12 +219,12 @@ __device__ __forceinline__ void load_rows_fast(const FetchArgs& a, const uint8_t
         // the source bytes are the layout: RGBA8 words, or RG8 pairs of a normal
         // map (8 bytes at 8 bx, 12 at 12 bx), at aligned addresses
 #pragma unroll
-        for (int r = 0; r < DIM; ++r, row += pitch)
+        for (int r = 0; r < NR; ++r, row += pitch)
 #pragma unroll
             for (int i = 0; i < L::WPR; ++i) out[r * L::WPR + i] = reinterpret_cast<const uint32_t*>(row)[i];
     } else {
 #pragma unroll
-        for (int r = 0; r < DIM; ++r, row += pitch) {
+        for (int r = 0; r < NR; ++r, row += pitch) {
             uint32_t t[DIM];
             if (a.fmt == FMT_RGBA8) {
 #pragma unroll
@@ -245,9 +245,11 @@ __device__ __forceinline__ void load_rows_fast(const FetchArgs& a, const uint8_t
 }
 
 // Load a block's texels in the class's layout (loads only; no conversion), so
-// that a caller can issue the next block's loads before encoding this one.
-template <int DIM, int CH>
-__device__ __forceinline__ void load_block(const FetchArgs& a, int bx, int by, uint32_t p[Layout<DIM, CH>::WORDS]) {
+// that a caller can issue the next block's loads before encoding this one:
+// NR rows from row r0 of the block (all of them by default).
+template <int DIM, int CH, int NR = DIM>
+__device__ __forceinline__ void load_block(const FetchArgs& a, int bx, int by, uint32_t p[NR * Layout<DIM, CH>::WPR],
+                                           int r0 = 0) {
     using L = Layout<DIM, CH>;
     const int x0 = bx * DIM, y0 = by * DIM;
     const int bpp = bytes_per_texel(a.fmt);
@@ -256,17 +258,18 @@ __device__ __forceinline__ void load_block(const FetchArgs& a, int bx, int by, u
     const bool fast = a.aligned && (x0 + DIM <= a.width) && (y0 + DIM <= a.height) &&
                       (((x0 + DIM) * bpp + 3) & ~3) <= a.width * bpp;
     if (fast) {
-        load_rows_fast<DIM, CH>(a, a.src + (size_t)(uint32_t)y0 * (uint32_t)a.row_pitch + uint32_t(x0 * bpp), p);
+        load_rows_fast<DIM, CH, NR>(
+            a, a.src + (size_t)(uint32_t)(y0 + r0) * (uint32_t)a.row_pitch + uint32_t(x0 * bpp), p);
     } else {
         if (CH != CH_ALL) {
 #pragma unroll
-            for (int i = 0; i < L::WORDS; ++i) p[i] = 0u;
+            for (int i = 0; i < NR * L::WPR; ++i) p[i] = 0u;
         }
 #pragma unroll
-        for (int r = 0; r < DIM; ++r)
+        for (int r = 0; r < NR; ++r)
 #pragma unroll
             for (int i = 0; i < DIM; ++i) {
-                const uint32_t t = load_texel_guarded(a, x0 + i, y0 + r);
+                const uint32_t t = load_texel_guarded(a, x0 + i, y0 + r0 + r);
                 if (CH == CH_ALL) {
                     p[r * DIM + i] = t;
                 } else {
@@ -283,12 +286,13 @@ __device__ __forceinline__ void load_block(const FetchArgs& a, int bx, int by, u
 // Layout -> floats (x255 scale) through LDS tables: `colour` converts the colour
 // channels (sRGB or UNORM), `unorm` the alpha.  Only the class's channels are
 // converted (the others are known constants to the encoder, see encode_block).
-// or_mask (CH_ALL only) applies the normal-map override to the words.
-template <int DIM, int CH>
-__device__ __forceinline__ void unpack_block(const uint32_t p[Layout<DIM, CH>::WORDS], uint32_t or_mask,
-                                             const float* colour, const float* unorm, float x[DIM * DIM][4]) {
+// or_mask (CH_ALL only) applies the normal-map override to the words.  NR rows
+// of texels (all of the block's by default).
+template <int DIM, int CH, int NR = DIM>
+__device__ __forceinline__ void unpack_block(const uint32_t p[NR * Layout<DIM, CH>::WPR], uint32_t or_mask,
+                                             const float* colour, const float* unorm, float x[NR * DIM][4]) {
 #pragma unroll
-    for (int k = 0; k < DIM * DIM; ++k) {
+    for (int k = 0; k < NR * DIM; ++k) {
         if (CH == CH_ALL) {
             const uint32_t w = p[k] | or_mask;
 #pragma unroll
@@ -381,14 +385,197 @@ __device__ __forceinline__ Block128 pack_block(const uint32_t e0q[4], const uint
 // commented out (:170-227, :514).
 enum : int { METHOD_PCA = 0, METHOD_MAX_ACCUMULATION = 1 };
 
+// Texels summed by one chain: a 6x6 block's sums run over its two halves
+// (texels 0-17, rows 0-2, and 18-35, rows 3-5) separately and add the two, so
+// that each half can be summed by its own lane (encode_block_pair); 4x4 blocks
+// sum all 16 in one chain.
+template <int DIM>
+struct SumHalf { static constexpr int value = DIM == 6 ? 18 : DIM * DIM; };
+
+// The power iteration of eigen_vector (:93-106) on the covariance m, from
+// v0 = (0.26726, 0.80178, 0.53452, 0).  One iteration is v <- normalize(M (M v)),
+// after the test length(M v) < eps, which returns M v.  The first iteration is
+// taken as written (M v0, its test, then M (M v0)).  Past it, the test cannot
+// succeed: |M v|^2 = dot(v, S v) with S = M M is the Rayleigh quotient of the
+// positive semi-definite S at the unit v, and it never decreases along power
+// iterates.  The remaining seven iterations use S (formed once,
+// S_ij = dot(row i, row j)) and the last one normalises.  Only the direction
+// feeds the next product, so the iterate is rescaled by a power of two (exact)
+// before products 1, 4 and 7 (RESCALE = 3; 1 rescales before every product,
+// for register budgets that need it): three products from a largest component
+// in [0.5, 1) stay in the normal range (each grows it by at most
+// 4 max|S_ij| < 8e10 and shrinks it by at most half the first test's 1e-10
+// bound on the Rayleigh quotient), and scaling by powers of two commutes with
+// rounding, so fewer rescales change no bit.
+template <int NC, int RESCALE>
+__device__ __forceinline__ void principal_axis(const float m[4][4], float v[4]) {
+    float s2[4][4];
+#pragma unroll
+    for (int i = 0; i < NC; ++i)
+#pragma unroll
+        for (int j = i; j < NC; ++j) s2[i][j] = s2[j][i] = dotn<NC>(m[i], m[j]);
+    const float v0[4] = {0.26726f, 0.80178f, 0.53452f, 0.0f};
+    float u[4], w[4];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) u[i] = dotn<NC>(m[i], v0);
+    if (is_short(dotn<NC>(u, u))) {
+#pragma unroll
+        for (int i = 0; i < NC; ++i) v[i] = u[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < NC; ++i) w[i] = dotn<NC>(m[i], u);
+#pragma unroll
+        for (int it = 1; it < 8; ++it) {
+            if ((it - 1) % RESCALE == 0) {
+                float mx = fabsf(w[0]);
+#pragma unroll
+                for (int i = 1; i < NC; ++i) mx = fmaxf(mx, fabsf(w[i]));
+                const int e = exponent_of(mx);
+#pragma unroll
+                for (int i = 0; i < NC; ++i) v[i] = ldexp_(w[i], -e);
+            } else {
+#pragma unroll
+                for (int i = 0; i < NC; ++i) v[i] = w[i];
+            }
+#pragma unroll
+            for (int i = 0; i < NC; ++i) w[i] = dotn<NC>(s2[i], v);
+        }
+        const float sc = rsqrt_nr(dotn<NC>(w, w));
+#pragma unroll
+        for (int i = 0; i < NC; ++i) v[i] = w[i] * sc;
+    }
+#pragma unroll
+    for (int i = NC; i < 4; ++i) v[i] = 0.0f;
+}
+
+// The covariance matrix (:149-162) from the upper-triangle sums of products.
+template <int NC, int N>
+__device__ __forceinline__ void covariance_matrix(const float cov[NC * (NC + 1) / 2], float m[4][4]) {
+    constexpr float rcp_n1 = 1.0f / float(N - 1);
+    int p = 0;
+#pragma unroll
+    for (int a = 0; a < NC; ++a)
+#pragma unroll
+        for (int b2 = a; b2 < NC; ++b2, ++p) m[a][b2] = m[b2][a] = cov[p] * rcp_n1;
+}
+
+// Sums of products of the differences from the mean over texels [K0, K1) of x:
+// one fused multiply-add chain per upper-triangle entry, starting with a plain
+// product.
+template <int NC, int K0, int K1, int NX>
+__device__ __forceinline__ void covariance_chain(const float x[NX][4], const float mean[4],
+                                                 float cov[NC * (NC + 1) / 2]) {
+#pragma unroll
+    for (int k = K0; k < K1; ++k) {
+        float d[4];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) d[c] = x[k][c] - mean[c];
+        int p = 0;
+#pragma unroll
+        for (int a = 0; a < NC; ++a)
+#pragma unroll
+            for (int b2 = a; b2 < NC; ++b2, ++p) cov[p] = (k == K0) ? d[a] * d[b2] : fma_(d[a], d[b2], cov[p]);
+    }
+}
+
+// find_min_max's endpoints (:108-137) from the extreme projections lo, hi of
+// the texels on v: e = clamp(v * (t - dot(mean, v)) + mean), the alpha override
+// of RGB mode, the rounded endpoints (encode_color :233-245) and the swap flag.
+// The shader swaps e0 and e1 when e0's rounded RGB sum is the larger
+// (:125-130).  Swapping only reverses each endpoint byte pair and negates
+// e1 - e0; negation is exact and round-to-nearest is symmetric in sign, so
+// every later quantity of the swapped block is the unswapped one with its sign
+// flipped.  The swap is carried as that flag instead of moving values.
+template <bool ALPHA, int NC>
+__device__ __forceinline__ bool endpoints(const float v[4], float lo, float hi, const float mean[4], float e0[4],
+                                          float e1[4], uint32_t e0q[4], uint32_t e1q[4]) {
+    const float mv = dotn<NC>(mean, v);
+    lo = lo - mv;
+    hi = hi - mv;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) e0[c] = e1[c] = 255.0f;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        e0[c] = clamp255(fma_(v[c], lo, mean[c]));
+        e1[c] = clamp255(fma_(v[c], hi, mean[c]));
+    }
+    const float s0 = (rintf(e0[0]) + rintf(e0[1])) + rintf(e0[2]);
+    const float s1 = (rintf(e1[0]) + rintf(e1[1])) + rintf(e1[2]);
+    if (!ALPHA) {
+        e0[3] = 255.0f;
+        e1[3] = 255.0f;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        e0q[c] = uint32_t(rintf(e0[c]));
+        e1q[c] = uint32_t(rintf(e1[c]));
+    }
+    return s0 > s1;
+}
+
+// The weight axis of calculate_normal_weights (:316-372): k = e1 - e0 over the
+// KC channels that differ, scaled to unit length.  The weight of grid point i
+// is dot(k, sample_i - e0) rescaled to [0, 1] by its minimum and range over
+// the grid, so the constant dot(k, e0) cancels: the projections
+// p_i = dot(k, sample_i) are rescaled instead.  |e1 - e0| < 1e-5 gives
+// all-zero weights (:323-330); a zero scale gives the same fields without a
+// branch: every projection is +-0, the range is 0, so each (p - min) * 1e5 is
+// +-0 and rounds to weight 0.  With the swap, the scale is negated (e1 - e0 of
+// the swapped block).
+template <int KC>
+__device__ __forceinline__ void weight_axis(const float e0[4], const float e1[4], bool swap, float k[4]) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) k[c] = 0.0f;
+#pragma unroll
+    for (int c = 0; c < KC; ++c) k[c] = e1[c] - e0[c];
+    const float kk = dotn<KC>(k, k);
+    float s = is_short(kk) ? 0.0f : rsqrt_nr(kk);
+    s = swap ? -s : s;
+#pragma unroll
+    for (int c = 0; c < KC; ++c) k[c] = k[c] * s;
+}
+
+// 6x6 sample projections of grid points [G0, G0 + NG) from texel projections
+// pt (texel index minus T0): each sample is a fixed 4-tap combination
+// (sample_texel :307-314 with idx_grids / wt_grids :268-304), and so is its
+// projection.
+template <int G0, int NG, int T0, int NT>
+__device__ __forceinline__ void grid6_samples(const float pt[NT], float proj[NG]) {
+    constexpr float tap_w[3] = {0.444f, 0.222f, 0.111f};
+#pragma unroll
+    for (int j = 0; j < NG; ++j) {
+        const int i = G0 + j;
+        float smp = pt[grid6_tap_index(i, 0) - T0] * tap_w[grid6_tap_far(i, 0)];
+        smp = fma_(pt[grid6_tap_index(i, 1) - T0], tap_w[grid6_tap_far(i, 1)], smp);
+        smp = fma_(pt[grid6_tap_index(i, 2) - T0], tap_w[grid6_tap_far(i, 2)], smp);
+        proj[j] = fma_(pt[grid6_tap_index(i, 3) - T0], tap_w[grid6_tap_far(i, 3)], smp);
+    }
+}
+
+// quantize_weights (:374-382) of rescaled projections: (w - min) * invlen is at
+// most 1 + 2^-22, so round(w * range) never exceeds range and the clamp of
+// quantize_weight (:258) never acts.  round-half-even(y) for 0 <= y < 2^22 is
+// the low bits of y + 1.5 * 2^23 (the sum's unit in the last place is 1);
+// doubled, its low five bits are 2 * round(y).
+template <bool ALPHA, int NG>
+__device__ __forceinline__ void quantize_weights(const float proj[NG], float minw, float maxw, uint32_t wsh[NG]) {
+    constexpr uint32_t range = ALPHA ? 5u : 11u;
+    const float invlen = 1.0f / fmaxf(0.00001f, maxw - minw);
+#pragma unroll
+    for (int i = 0; i < NG; ++i) {
+        const float wn = (proj[i] - minw) * invlen;
+        wsh[i] = __float_as_uint(wn * float(range) + 12582912.0f) << 1;
+    }
+}
+
 // RESCALE: the power iteration's rescale interval, 3 or 1 (every product); both
-// give the same blocks (see the power iteration below).
+// give the same blocks (see principal_axis).
 template <int DIM, bool ALPHA, int NC = 4, int METHOD = METHOD_PCA, int RESCALE = 3>
 __device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], const PackTables* pack) {
     static_assert(NC >= 2 && NC <= 4, "two to four active channels");
     constexpr int N = DIM * DIM;
+    constexpr int H = SumHalf<DIM>::value;
     constexpr float rcp_n = 1.0f / float(N);
-    constexpr float rcp_n1 = 1.0f / float(N - 1);
 
     // mean (principal_component_analysis :142-147)
     float mean[4] = {255.0f, 255.0f, 255.0f, 255.0f};
@@ -396,7 +583,13 @@ __device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], co
     for (int c = 0; c < NC; ++c) {
         float r = x[0][c];
 #pragma unroll
-        for (int k = 1; k < N; ++k) r = r + x[k][c];
+        for (int k = 1; k < H; ++k) r = r + x[k][c];
+        if (H < N) {
+            float r2 = x[H][c];
+#pragma unroll
+            for (int k = H + 1; k < N; ++k) r2 = r2 + x[k][c];
+            r = r + r2;
+        }
         mean[c] = r * rcp_n;
     }
 
@@ -441,85 +634,23 @@ __device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], co
 #pragma unroll
             for (int j = 0; j < NC; ++j) v[j] = v[j] * sc;
         }
+#pragma unroll
+        for (int i = NC; i < 4; ++i) v[i] = 0.0f;
     } else {
-        // covariance (:149-162): upper triangle of the active channels, one fused
-        // multiply-add chain per entry over the texels
+        // covariance (:149-162): upper triangle of the active channels
         constexpr int NP = NC * (NC + 1) / 2;
         float cov[NP];
+        covariance_chain<NC, 0, H, N>(x, mean, cov);
+        if (H < N) {
+            float cov2[NP];
+            covariance_chain<NC, H, N, N>(x, mean, cov2);
 #pragma unroll
-        for (int k = 0; k < N; ++k) {
-            float d[4];
-#pragma unroll
-            for (int c = 0; c < NC; ++c) d[c] = x[k][c] - mean[c];
-            int p = 0;
-#pragma unroll
-            for (int a = 0; a < NC; ++a)
-#pragma unroll
-                for (int b2 = a; b2 < NC; ++b2, ++p) cov[p] = (k == 0) ? d[a] * d[b2] : fma_(d[a], d[b2], cov[p]);
+            for (int p = 0; p < NP; ++p) cov[p] = cov[p] + cov2[p];
         }
-#pragma unroll
-        for (int p = 0; p < NP; ++p) cov[p] = cov[p] * rcp_n1;
         float m[4][4];
-        {
-            int p = 0;
-#pragma unroll
-            for (int a = 0; a < NC; ++a)
-#pragma unroll
-                for (int b2 = a; b2 < NC; ++b2, ++p) m[a][b2] = m[b2][a] = cov[p];
-        }
-
-        // eigen_vector (:93-106).  One iteration is v <- normalize(M (M v)), after
-        // the test length(M v) < eps, which returns M v.  The first iteration is
-        // taken as written (M v0, its test, then M (M v0)).  Past it, the test
-        // cannot succeed: |M v|^2 = dot(v, S v) with S = M M is the Rayleigh
-        // quotient of the positive semi-definite S at the unit v, and it never
-        // decreases along power iterates.  The remaining seven iterations use S
-        // (formed once, S_ij = dot(row i, row j)) and the last one normalises.
-        // Only the direction feeds the next product, so the iterate is rescaled
-        // by a power of two (exact) before products 1, 4 and 7 (RESCALE = 3; 1
-        // rescales before every product, for register budgets that need it): three products
-        // from a largest component in [0.5, 1) stay in the normal range (each
-        // grows it by at most 4 max|S_ij| < 8e10 and shrinks it by at most half
-        // the first test's 1e-10 bound on the Rayleigh quotient), and scaling by
-        // powers of two commutes with rounding, so fewer rescales change no bit.
-        float s2[4][4];
-#pragma unroll
-        for (int i = 0; i < NC; ++i)
-#pragma unroll
-            for (int j = i; j < NC; ++j) s2[i][j] = s2[j][i] = dotn<NC>(m[i], m[j]);
-        const float v0[4] = {0.26726f, 0.80178f, 0.53452f, 0.0f};
-        float u[4], w[4];
-#pragma unroll
-        for (int i = 0; i < NC; ++i) u[i] = dotn<NC>(m[i], v0);
-        if (is_short(dotn<NC>(u, u))) {
-#pragma unroll
-            for (int i = 0; i < NC; ++i) v[i] = u[i];
-        } else {
-#pragma unroll
-            for (int i = 0; i < NC; ++i) w[i] = dotn<NC>(m[i], u);
-#pragma unroll
-            for (int it = 1; it < 8; ++it) {
-                if ((it - 1) % RESCALE == 0) {
-                    float mx = fabsf(w[0]);
-#pragma unroll
-                    for (int i = 1; i < NC; ++i) mx = fmaxf(mx, fabsf(w[i]));
-                    const int e = exponent_of(mx);
-#pragma unroll
-                    for (int i = 0; i < NC; ++i) v[i] = ldexp_(w[i], -e);
-                } else {
-#pragma unroll
-                    for (int i = 0; i < NC; ++i) v[i] = w[i];
-                }
-#pragma unroll
-                for (int i = 0; i < NC; ++i) w[i] = dotn<NC>(s2[i], v);
-            }
-            const float sc = rsqrt_nr(dotn<NC>(w, w));
-#pragma unroll
-            for (int i = 0; i < NC; ++i) v[i] = w[i] * sc;
-        }
+        covariance_matrix<NC, N>(cov, m);
+        principal_axis<NC, RESCALE>(m, v);
     }
-#pragma unroll
-    for (int i = NC; i < 4; ++i) v[i] = 0.0f;
 
     // find_min_max (:108-137): t_k = dot(texel_k - mean, v), taken as
     // dot(texel_k, v) - dot(mean, v); rounding is monotonic, so the minimum and
@@ -531,91 +662,121 @@ __device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], co
         lo = fminf(lo, t);
         hi = fmaxf(hi, t);
     }
-    const float mv = dotn<NC>(mean, v);
-    lo = lo - mv;
-    hi = hi - mv;
-    float e0[4] = {255.0f, 255.0f, 255.0f, 255.0f}, e1[4] = {255.0f, 255.0f, 255.0f, 255.0f};
+    float e0[4], e1[4];
+    uint32_t e0q[4], e1q[4];
+    const bool swap = endpoints<ALPHA, NC>(v, lo, hi, mean, e0, e1, e0q, e1q);
+
+    // calculate_normal_weights (:316-372) + quantize_weights (:374-382)
+    constexpr int KC = (NC == 4 && !ALPHA) ? 3 : NC;  // RGB mode: endpoint alphas are equal
+    float k[4];
+    weight_axis<KC>(e0, e1, swap, k);
+    float proj[16];
+    if (DIM == 6) {
+        float pt[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) pt[j] = dotn<KC>(k, x[j]);
+        grid6_samples<0, 16, 0, N>(pt, proj);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) proj[i] = dotn<KC>(k, x[i]);
+    }
+    float minw = 1e31f, maxw = -1e31f;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        minw = fminf(proj[i], minw);
+        maxw = fmaxf(proj[i], maxw);
+    }
+    uint32_t wsh[16];  // 2 * quantised weight in the low five bits
+    quantize_weights<ALPHA, 16>(proj, minw, maxw, wsh);
+    return pack_block<ALPHA>(e0q, e1q, wsh, pack, swap);
+}
+
+// ---------------------------------------------------------------------------
+// Two lanes per 6x6 block: lanes 2j and 2j+1 of a wave share one block, lane
+// `half` holding texel rows 3 half .. 3 half + 2 (texels 18 half ..
+// 18 half + 17).  Each lane sums, projects and samples its own half; the
+// halves are combined by one exchange between the two lanes (DPP quad
+// permutation [1, 0, 3, 2]):
+//   * sums: own + other.  IEEE addition is commutative, so both lanes hold the
+//     same bits, and they equal the one-lane schedule's (half 0) + (half 1);
+//   * minimum and maximum: exact and order-free;
+//   * the grid: grid rows 0-1 sample texel rows 0-2 only and grid rows 2-3
+//     texel rows 3-5 only, so each lane samples its own eight grid points, at
+//     the same texel offsets within its half; their quantised weights (4 bits
+//     each) are exchanged as one word.
+// The serial part (power iteration, endpoints, integer packing) runs on both
+// lanes with identical inputs.  Blocks equal encode_block's bit for bit.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t pair_other(uint32_t v) {
+    return uint32_t(__builtin_amdgcn_mov_dpp(int(v), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float pair_other(float v) { return __uint_as_float(pair_other(__float_as_uint(v))); }
+
+template <bool ALPHA, int NC = 4, int RESCALE = 3>
+__device__ __forceinline__ Block128 encode_block_pair(const float x[18][4], int half, const PackTables* pack) {
+    static_assert(NC >= 2 && NC <= 4, "two to four active channels");
+    constexpr int N = 36, NH = 18;
+    constexpr float rcp_n = 1.0f / float(N);
+
+    float mean[4] = {255.0f, 255.0f, 255.0f, 255.0f};
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-        e0[c] = clamp255(fma_(v[c], lo, mean[c]));
-        e1[c] = clamp255(fma_(v[c], hi, mean[c]));
+        float r = x[0][c];
+#pragma unroll
+        for (int k = 1; k < NH; ++k) r = r + x[k][c];
+        mean[c] = (r + pair_other(r)) * rcp_n;
     }
-    // The shader swaps e0 and e1 when e0's rounded RGB sum is the larger
-    // (:125-130).  Swapping only reverses each endpoint byte pair and negates
-    // e1 - e0; negation is exact and round-to-nearest is symmetric in sign, so
-    // every later quantity of the swapped block is the unswapped one with its
-    // sign flipped.  The swap is carried as that flag instead of moving values.
-    const float s0 = (rintf(e0[0]) + rintf(e0[1])) + rintf(e0[2]);
-    const float s1 = (rintf(e1[0]) + rintf(e1[1])) + rintf(e1[2]);
-    const bool swap = s0 > s1;
-    if (!ALPHA) {
-        e0[3] = 255.0f;
-        e1[3] = 255.0f;
-    }
+    constexpr int NP = NC * (NC + 1) / 2;
+    float cov[NP];
+    covariance_chain<NC, 0, NH, NH>(x, mean, cov);
+#pragma unroll
+    for (int p = 0; p < NP; ++p) cov[p] = cov[p] + pair_other(cov[p]);
+    float m[4][4];
+    covariance_matrix<NC, N>(cov, m);
+    float v[4];
+    principal_axis<NC, RESCALE>(m, v);
 
-    // encode_color (:233-245)
+    float lo = 1e31f, hi = -1e31f;
+#pragma unroll
+    for (int k = 0; k < NH; ++k) {
+        const float t = dotn<NC>(x[k], v);
+        lo = fminf(lo, t);
+        hi = fmaxf(hi, t);
+    }
+    lo = fminf(lo, pair_other(lo));
+    hi = fmaxf(hi, pair_other(hi));
+    float e0[4], e1[4];
     uint32_t e0q[4], e1q[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        e0q[c] = uint32_t(rintf(e0[c]));
-        e1q[c] = uint32_t(rintf(e1[c]));
-    }
+    const bool swap = endpoints<ALPHA, NC>(v, lo, hi, mean, e0, e1, e0q, e1q);
 
-    // calculate_normal_weights (:316-372) + quantize_weights (:374-382).  The
-    // weight of grid point i is dot(k, sample_i - e0) rescaled to [0, 1] by its
-    // minimum and range over the grid, so the constant dot(k, e0) cancels: the
-    // projections p_i = dot(k, sample_i) are rescaled instead.  A 6x6 sample is
-    // a fixed 4-tap combination of texels, and so is its projection.
-    constexpr uint32_t range = ALPHA ? 5u : 11u;
-    constexpr int KC = (NC == 4 && !ALPHA) ? 3 : NC;  // RGB mode: endpoint alphas are equal
-    uint32_t wsh[16];  // 2 * quantised weight in the low five bits
-    float k[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    constexpr int KC = (NC == 4 && !ALPHA) ? 3 : NC;
+    float k[4];
+    weight_axis<KC>(e0, e1, swap, k);
+    float pt[NH];
 #pragma unroll
-    for (int c = 0; c < KC; ++c) k[c] = e1[c] - e0[c];
-    const float kk = dotn<KC>(k, k);
-    // |e1 - e0| < 1e-5 gives all-zero weights (:323-330).  A zero scale gives
-    // the same fields without a branch: every projection is +-0, the range is 0,
-    // so each (p - min) * 1e5 is +-0 and rounds to weight 0.  With the swap, the
-    // scale is negated (e1 - e0 of the swapped block).
-    {
-        float s = is_short(kk) ? 0.0f : rsqrt_nr(kk);
-        s = swap ? -s : s;
+    for (int j = 0; j < NH; ++j) pt[j] = dotn<KC>(k, x[j]);
+    float proj[8];
+    grid6_samples<0, 8, 0, NH>(pt, proj);  // grid rows 0-1 of half 0 = grid rows 2-3 of half 1
+    float minw = 1e31f, maxw = -1e31f;
 #pragma unroll
-        for (int c = 0; c < KC; ++c) k[c] = k[c] * s;
-        float proj[16];
-        if (DIM == 6) {
-            float pt[N];
+    for (int i = 0; i < 8; ++i) {
+        minw = fminf(proj[i], minw);
+        maxw = fmaxf(proj[i], maxw);
+    }
+    minw = fminf(minw, pair_other(minw));
+    maxw = fmaxf(maxw, pair_other(maxw));
+    uint32_t own[8];
+    quantize_weights<ALPHA, 8>(proj, minw, maxw, own);
+    uint32_t packed = 0;  // the eight quantised weights, 4 bits each
 #pragma unroll
-            for (int j = 0; j < N; ++j) pt[j] = dotn<KC>(k, x[j]);
-            constexpr float tap_w[3] = {0.444f, 0.222f, 0.111f};
+    for (int i = 0; i < 8; ++i) packed |= ((own[i] >> 1) & 15u) << (4 * i);
+    const uint32_t other = pair_other(packed);
+    const uint32_t first = half ? other : packed, second = half ? packed : other;
+    uint32_t wsh[16];
 #pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                float smp = pt[grid6_tap_index(i, 0)] * tap_w[grid6_tap_far(i, 0)];
-                smp = fma_(pt[grid6_tap_index(i, 1)], tap_w[grid6_tap_far(i, 1)], smp);
-                smp = fma_(pt[grid6_tap_index(i, 2)], tap_w[grid6_tap_far(i, 2)], smp);
-                proj[i] = fma_(pt[grid6_tap_index(i, 3)], tap_w[grid6_tap_far(i, 3)], smp);
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < 16; ++i) proj[i] = dotn<KC>(k, x[i]);
-        }
-        float minw = 1e31f, maxw = -1e31f;
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            minw = fminf(proj[i], minw);
-            maxw = fmaxf(proj[i], maxw);
-        }
-        // (w - min) * invlen is at most 1 + 2^-22, so round(w * range) never
-        // exceeds range and the clamp of quantize_weight (:258) never acts.
-        // round-half-even(y) for 0 <= y < 2^22 is the low bits of y + 1.5 * 2^23
-        // (the sum's unit in the last place is 1); doubled, its low five bits
-        // are 2 * round(y).
-        const float invlen = 1.0f / fmaxf(0.00001f, maxw - minw);
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const float wn = (proj[i] - minw) * invlen;
-            wsh[i] = __float_as_uint(wn * float(range) + 12582912.0f) << 1;
-        }
+    for (int i = 0; i < 8; ++i) {
+        wsh[i] = ((first >> (4 * i)) & 15u) << 1;
+        wsh[8 + i] = ((second >> (4 * i)) & 15u) << 1;
     }
     return pack_block<ALPHA>(e0q, e1q, wsh, pack, swap);
 }

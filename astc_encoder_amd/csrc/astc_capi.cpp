@@ -138,6 +138,61 @@ int astc_encode(astc_ctx* ctx, const astc_image_desc* desc, const void* src_dev,
     return guarded([&]() -> int { return encode_one(ctx, desc, src_dev, dst_dev, stream); });
 }
 
+int astc_encode_tuned(astc_ctx* ctx, const astc_image_desc* desc, const void* src_dev, void* dst_dev,
+                      const astc_plain_tuning* tuning, void* stream) {
+    return guarded([&]() -> int {
+        if (!tuning) return fail(ASTC_ERR_INVALID_ARG, "tuning is NULL");
+        return encode_one(ctx, desc, src_dev, dst_dev, stream, tuning);
+    });
+}
+
+int astc_plain_autotune(astc_ctx* ctx, const astc_image_desc* desc, const void* src_dev, void* dst_dev, void* stream,
+                        int reps, astc_plain_tuning* best) {
+    return guarded([&]() -> int {
+        if (!ctx || !desc || !best) return fail(ASTC_ERR_INVALID_ARG, "NULL argument");
+        if (reps < 1) reps = 1;
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        hipEvent_t t0, t1;
+        hipError_t e;
+        if ((e = hipSetDevice(ctx->device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
+        if ((e = hipEventCreate(&t0)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+        if ((e = hipEventCreate(&t1)) != hipSuccess) {
+            (void)hipEventDestroy(t0);
+            return hip_fail(e, "hipEventCreate");
+        }
+        float best_ms = 1e30f;
+        int rc = ASTC_OK;
+        for (int lanes : {1, 2}) {
+            if (lanes == 2 && (desc->block_dim != 6 || (desc->flags & ASTC_FLAG_MAX_ACCUMULATION))) continue;
+            for (int bpl : {1, 2, 4, 8})
+                for (int pf : {0, 1}) {
+                    if (rc != ASTC_OK) break;
+                    const astc_plain_tuning t{lanes, bpl, pf, 0};
+                    if ((rc = encode_one(ctx, desc, src_dev, dst_dev, stream, &t)) != ASTC_OK) break;  // warm-up
+                    if ((e = hipEventRecord(t0, s)) != hipSuccess) {
+                        rc = hip_fail(e, "hipEventRecord");
+                        break;
+                    }
+                    for (int r = 0; r < reps && rc == ASTC_OK; ++r) rc = encode_one(ctx, desc, src_dev, dst_dev, stream, &t);
+                    if (rc != ASTC_OK) break;
+                    if ((e = hipEventRecord(t1, s)) != hipSuccess || (e = hipEventSynchronize(t1)) != hipSuccess) {
+                        rc = hip_fail(e, "timing event");
+                        break;
+                    }
+                    float ms = 0.f;
+                    (void)hipEventElapsedTime(&ms, t0, t1);
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        *best = t;
+                    }
+                }
+        }
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        return rc;
+    });
+}
+
 int astc_decode(astc_ctx* ctx, int width, int height, int block_dim_x, int block_dim_y, const void* blocks_dev,
                 void* rgba_dev, void* stream) {
     return guarded([&]() -> int {
@@ -149,8 +204,20 @@ int astc_decode(astc_ctx* ctx, int width, int height, int block_dim_x, int block
 
 namespace astc {
 
-int encode_one(astc_ctx* ctx, const astc_image_desc* desc, const void* src_dev, void* dst_dev, void* stream) {
+int encode_one(astc_ctx* ctx, const astc_image_desc* desc, const void* src_dev, void* dst_dev, void* stream,
+               const astc_plain_tuning* tuning) {
     if (!ctx) return fail(ASTC_ERR_INVALID_ARG, "ctx is NULL");
+    PlainTuning pt{};
+    if (tuning) {
+        const int bpl = tuning->blocks_per_lane;
+        if ((tuning->lanes_per_block != 1 && tuning->lanes_per_block != 2) ||
+            !(bpl == 0 || bpl == 1 || bpl == 2 || bpl == 4 || bpl == 8) ||
+            (tuning->prefetch != 0 && tuning->prefetch != 1))
+            return fail(ASTC_ERR_INVALID_ARG, "plain tuning out of range");
+        pt.lanes_per_block = tuning->lanes_per_block;
+        pt.blocks_per_lane = bpl;
+        pt.prefetch = tuning->prefetch;
+    }
     ImageGeom g;
     int rc = check_desc(desc, &g);
     if (rc != ASTC_OK) return rc;
@@ -158,7 +225,8 @@ int encode_one(astc_ctx* ctx, const astc_image_desc* desc, const void* src_dev, 
     if (reinterpret_cast<uintptr_t>(dst_dev) & 15u) return fail(ASTC_ERR_INVALID_ARG, "dst must be 16-byte aligned");
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-    e = launch_encode_plain(ctx->d_tables, *desc, g, src_dev, dst_dev, static_cast<hipStream_t>(stream), ctx->num_cus);
+    e = launch_encode_plain(ctx->d_tables, *desc, g, src_dev, dst_dev, static_cast<hipStream_t>(stream), ctx->num_cus,
+                            tuning ? &pt : nullptr);
     if (e != hipSuccess) return hip_fail(e, "encode launch");
     return ASTC_OK;
 }

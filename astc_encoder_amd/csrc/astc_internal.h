@@ -77,7 +77,8 @@ auto guarded(F&& f) -> decltype(f()) {
 }
 
 // Bodies of astc_encode / astc_decode (the entry points run them under guarded).
-int encode_one(astc_ctx* ctx, const astc_image_desc* desc, const void* src_dev, void* dst_dev, void* stream);
+int encode_one(astc_ctx* ctx, const astc_image_desc* desc, const void* src_dev, void* dst_dev, void* stream,
+               const astc_plain_tuning* tuning = nullptr);
 int decode_one(astc_ctx* ctx, int width, int height, int block_dim_x, int block_dim_y, const void* blocks_dev,
                void* rgba_dev, void* stream);
 
@@ -167,9 +168,17 @@ hipError_t launch_downsample(const uint8_t* src, int w, int h, int bpp, uint8_t*
 hipError_t launch_decode(const TritQuintDecodeTable* d_tables, const void* blocks, void* rgba, int width, int height,
                          int bw, int bh, hipStream_t stream);
 
-// Plain per-texture launch (one lane per block) on a device with num_cus CUs.
+// Plain per-texture launch on a device with num_cus CUs.  tuning: NULL for
+// the default (default_plain_tuning); blocks_per_lane 0 picks the count.
+struct PlainTuning {
+    int lanes_per_block;  // 1, or 2 (6x6 principal-component encodes only; others run 1)
+    int blocks_per_lane;  // blocks per lane per workgroup: 1, 2, 4, 8, or 0 for the library's choice
+    int prefetch;         // 1: the next block's loads are issued before the current block is encoded
+};
 hipError_t launch_encode_plain(const DeviceTables* d_tables, const astc_image_desc& d, const ImageGeom& g,
-                               const void* src, void* dst, hipStream_t stream, int num_cus);
-int plain_blocks_per_lane(int64_t nblocks, int dim, int num_cus);
+                               const void* src, void* dst, hipStream_t stream, int num_cus,
+                               const PlainTuning* tuning = nullptr);
+int plain_blocks_per_lane(int64_t nblocks, int dim, int lanes_per_block, int num_cus);
+PlainTuning default_plain_tuning(const astc_image_desc& d, int64_t nblocks, int num_cus);
 
 }  // namespace astc

@@ -8,10 +8,12 @@
 // useful VALU work.  A workgroup of 256 lanes covers BPL x 256 consecutive
 // blocks in row-major block order: in step k lane t encodes block
 // base + 256 k + t, so the lanes of a wave read adjacent block rows of the
-// texture and write 16-byte blocks to consecutive addresses.  Each lane issues
-// the loads of its next block before encoding the current one, which hides the
-// HBM latency behind ~2k VALU instructions even at two waves per SIMD (the
-// 6x6 encoder holds 144 texel floats per lane).
+// texture and write 16-byte blocks to consecutive addresses.  By default a
+// lane loads its next block after storing the current one and the other waves
+// of the SIMD cover the load (measured faster than issuing the next block's
+// loads first, which stays a tuning); the 6x6 encoder holds 144 texel floats
+// per lane and runs two waves per SIMD.  encode_plain_pair_kernel is the
+// two-lanes-per-block 6x6 form (astc_plain_tuning.lanes_per_block = 2).
 #include "astc_internal.h"
 
 namespace astc {
@@ -57,11 +59,11 @@ __host__ __device__ inline int channel_class(int fmt, uint32_t flags, int width,
 // normal map through the two-channel class, which reads red and green only, and
 // the mask is zero for every other image, so only the four-channel
 // max-accumulation search sets it.
-template <int DIM, int SRGB, int CH, bool OVERRIDE = false>
-__device__ __forceinline__ void unpack_texels(const uint32_t p[Layout<DIM, CH>::WORDS], const FetchArgs& f,
-                                              const float* lut, float x[DIM * DIM][4]) {
+template <int DIM, int SRGB, int CH, bool OVERRIDE = false, int NR = DIM>
+__device__ __forceinline__ void unpack_texels(const uint32_t p[NR * Layout<DIM, CH>::WPR], const FetchArgs& f,
+                                              const float* lut, float x[NR * DIM][4]) {
     const float* colour = SRGB < 0 ? lut + ((f.flags & FLAG_SRGB) ? 0 : 256) : SRGB == 1 ? lut : lut + 256;
-    unpack_block<DIM, CH>(p, OVERRIDE ? f.or_mask : 0u, colour, lut + 256, x);
+    unpack_block<DIM, CH, NR>(p, OVERRIDE ? f.or_mask : 0u, colour, lut + 256, x);
 }
 
 __device__ __forceinline__ void store_block(uint32_t* dst, int id, const Block128& b) {
@@ -119,7 +121,7 @@ struct PlainWaves { static constexpr int value = DIM == 4 ? 4 : 2; };
 
 template <int DIM, bool ALPHA, bool SRGB, int CH, int METHOD>
 __global__ void __launch_bounds__(kPlainThreads)
-__attribute__((amdgpu_waves_per_eu(PlainWaves<DIM>::value))) encode_plain_kernel(PlainArgs a, int bpl) {
+__attribute__((amdgpu_waves_per_eu(PlainWaves<DIM>::value))) encode_plain_kernel(PlainArgs a, int bpl, int prefetch) {
     __shared__ float lut[512];
     __shared__ PackTables trit;
     stage_tables(a.tables, lut, &trit);
@@ -127,7 +129,61 @@ __attribute__((amdgpu_waves_per_eu(PlainWaves<DIM>::value))) encode_plain_kernel
     const int chunk = kPlainThreads * bpl;
     const int first = blockIdx.x * chunk;
     const int count = min(chunk, a.nblocks - first);
-    encode_chunk<DIM, ALPHA, SRGB ? 1 : 0, CH, METHOD>(a.f, a.dst, first, count, true, lut, &trit);
+    encode_chunk<DIM, ALPHA, SRGB ? 1 : 0, CH, METHOD>(a.f, a.dst, first, count, prefetch != 0, lut, &trit);
+}
+
+// ---------------------------------------------------------------------------
+// Two lanes per 6x6 block (encode_block_pair): lanes 2j and 2j+1 load, convert
+// and encode rows 0-2 and 3-5 of one block and each store 8 of its 16 bytes.
+// A workgroup of 256 lanes covers 128 blocks per step: in step k, lane pair j
+// encodes block first + 128 k + j.  Half the texels per lane lowers the
+// register budget (a third wave per SIMD) and halves the work of one wave
+// iteration (finer balance over the SIMDs); the serial part of the encode runs
+// on both lanes.
+// ---------------------------------------------------------------------------
+template <int CH>
+__device__ __forceinline__ void load_half_block(const FetchArgs& f, int id, int half,
+                                                uint32_t p[3 * Layout<6, CH>::WPR]) {
+    const int by = block_row_of(f, id);
+    load_block<6, CH, 3>(f, id - by * f.xblocks, by, p, 3 * half);
+}
+
+__device__ __forceinline__ void store_half_block(uint32_t* dst, int id, int half, const Block128& b) {
+    reinterpret_cast<uint2*>(dst)[2 * id + half] = half ? make_uint2(b.w[2], b.w[3]) : make_uint2(b.w[0], b.w[1]);
+}
+
+template <bool ALPHA, int SRGB, int CH, int RESCALE>
+__device__ __forceinline__ void encode_chunk_pair(const FetchArgs& f, uint32_t* dst, int first, int count,
+                                                  bool prefetch, const float* lut, const PackTables* trit) {
+    const int half = threadIdx.x & 1;
+    const int stride = blockDim.x >> 1;
+    int o = threadIdx.x >> 1;
+    if (o >= count) return;
+    uint32_t cur[3 * Layout<6, CH>::WPR];
+    load_half_block<CH>(f, first + o, half, cur);
+#pragma unroll 1
+    for (;; o += stride) {
+        const bool has_next = o + stride < count;
+        float x[18][4];
+        unpack_texels<6, SRGB, CH, false, 3>(cur, f, lut, x);
+        if (has_next && prefetch) load_half_block<CH>(f, first + o + stride, half, cur);
+        store_half_block(dst, first + o, half, encode_block_pair<ALPHA, CH, RESCALE>(x, half, trit));
+        if (!has_next) break;
+        if (!prefetch) load_half_block<CH>(f, first + o + stride, half, cur);
+    }
+}
+
+template <bool ALPHA, bool SRGB, int CH>
+__global__ void __launch_bounds__(kPlainThreads) __attribute__((amdgpu_waves_per_eu(3)))
+encode_plain_pair_kernel(PlainArgs a, int bpl, int prefetch) {
+    __shared__ float lut[512];
+    __shared__ PackTables trit;
+    stage_tables(a.tables, lut, &trit);
+    __syncthreads();
+    const int chunk = (kPlainThreads / 2) * bpl;
+    const int first = blockIdx.x * chunk;
+    const int count = min(chunk, a.nblocks - first);
+    encode_chunk_pair<ALPHA, SRGB ? 1 : 0, CH, 3>(a.f, a.dst, first, count, prefetch != 0, lut, &trit);
 }
 
 // ---------------------------------------------------------------------------
@@ -297,42 +353,66 @@ hipError_t launch_downsample(const uint8_t* src, int w, int h, int bpp, uint8_t*
 }
 
 template <int DIM, bool ALPHA, int CH, int METHOD = METHOD_PCA>
-static void launch_plain_ch(const PlainArgs& a, const dim3 grid, int bpl, hipStream_t stream) {
-    if (a.f.flags & FLAG_SRGB)
+static void launch_plain_ch(const PlainArgs& a, int64_t nblocks, const PlainTuning& t, hipStream_t stream) {
+    const bool srgb = a.f.flags & FLAG_SRGB;
+    if (DIM == 6 && METHOD == METHOD_PCA && t.lanes_per_block == 2) {
+        const int chunk = (kPlainThreads / 2) * t.blocks_per_lane;
+        const dim3 grid(unsigned((nblocks + chunk - 1) / chunk));
+        if (srgb)
+            hipLaunchKernelGGL((encode_plain_pair_kernel<ALPHA, true, CH>), grid, dim3(kPlainThreads), 0, stream, a,
+                               t.blocks_per_lane, t.prefetch);
+        else
+            hipLaunchKernelGGL((encode_plain_pair_kernel<ALPHA, false, CH>), grid, dim3(kPlainThreads), 0, stream, a,
+                               t.blocks_per_lane, t.prefetch);
+        return;
+    }
+    const int chunk = kPlainThreads * t.blocks_per_lane;
+    const dim3 grid(unsigned((nblocks + chunk - 1) / chunk));
+    if (srgb)
         hipLaunchKernelGGL((encode_plain_kernel<DIM, ALPHA, true, CH, METHOD>), grid, dim3(kPlainThreads), 0, stream, a,
-                           bpl);
+                           t.blocks_per_lane, t.prefetch);
     else
-        hipLaunchKernelGGL((encode_plain_kernel<DIM, ALPHA, false, CH, METHOD>), grid, dim3(kPlainThreads), 0, stream, a,
-                           bpl);
+        hipLaunchKernelGGL((encode_plain_kernel<DIM, ALPHA, false, CH, METHOD>), grid, dim3(kPlainThreads), 0, stream,
+                           a, t.blocks_per_lane, t.prefetch);
 }
 
 template <int DIM, bool ALPHA>
-static void launch_plain_t(const PlainArgs& a, int64_t nblocks, int bpl, hipStream_t stream) {
-    const dim3 grid(unsigned((nblocks + kPlainThreads * bpl - 1) / (kPlainThreads * bpl)));
-    if (a.f.flags & FLAG_MAX_ACCUMULATION) {  // optional search: four-channel encoder only
-        launch_plain_ch<DIM, ALPHA, CH_ALL, METHOD_MAX_ACCUMULATION>(a, grid, bpl, stream);
+static void launch_plain_t(const PlainArgs& a, int64_t nblocks, const PlainTuning& t, hipStream_t stream) {
+    if (a.f.flags & FLAG_MAX_ACCUMULATION) {  // optional search: four-channel, one-lane encoder only
+        PlainTuning one = t;
+        one.lanes_per_block = 1;
+        launch_plain_ch<DIM, ALPHA, CH_ALL, METHOD_MAX_ACCUMULATION>(a, nblocks, one, stream);
         return;
     }
     switch (channel_class(a.f.fmt, a.f.flags, a.f.width, a.f.height, DIM)) {
-    case CH_NORMAL: launch_plain_ch<DIM, ALPHA, CH_NORMAL>(a, grid, bpl, stream); break;
-    case CH_RGB8: launch_plain_ch<DIM, ALPHA, CH_RGB8>(a, grid, bpl, stream); break;
-    default: launch_plain_ch<DIM, ALPHA, CH_ALL>(a, grid, bpl, stream); break;
+    case CH_NORMAL: launch_plain_ch<DIM, ALPHA, CH_NORMAL>(a, nblocks, t, stream); break;
+    case CH_RGB8: launch_plain_ch<DIM, ALPHA, CH_RGB8>(a, nblocks, t, stream); break;
+    default: launch_plain_ch<DIM, ALPHA, CH_ALL>(a, nblocks, t, stream); break;
     }
 }
 
 // Blocks per lane for a plain launch: the fewest (1, 2, 4 or 8) that let every
-// workgroup be resident at once (6x6: two workgroups of four waves per CU, 4x4:
-// four), so that no second round of workgroup starts (table staging, first
-// fetch) trails the first and each lane's later blocks are prefetched.
-int plain_blocks_per_lane(int64_t nblocks, int dim, int num_cus) {
-    const int64_t slots = int64_t(num_cus) * (dim == 6 ? 2 : 4);
+// workgroup be resident at once (one-lane 6x6: two workgroups of four waves per
+// CU; two-lane 6x6: three; 4x4: four), so that no second round of workgroup
+// starts (table staging, first fetch) trails the first.
+int plain_blocks_per_lane(int64_t nblocks, int dim, int lanes_per_block, int num_cus) {
+    const int64_t slots = int64_t(num_cus) * (dim == 4 ? 4 : lanes_per_block == 2 ? 3 : 2);
+    const int64_t per_step = kPlainThreads / lanes_per_block;
     int bpl = 1;
-    while (bpl < 8 && (nblocks + int64_t(kPlainThreads) * bpl - 1) / (int64_t(kPlainThreads) * bpl) > slots) bpl *= 2;
+    while (bpl < 8 && (nblocks + per_step * bpl - 1) / (per_step * bpl) > slots) bpl *= 2;
     return bpl;
 }
 
+PlainTuning default_plain_tuning(const astc_image_desc& d, int64_t nblocks, int num_cus) {
+    PlainTuning t;
+    t.lanes_per_block = 1;
+    t.prefetch = 0;
+    t.blocks_per_lane = plain_blocks_per_lane(nblocks, d.block_dim, t.lanes_per_block, num_cus);
+    return t;
+}
+
 hipError_t launch_encode_plain(const DeviceTables* d_tables, const astc_image_desc& d, const ImageGeom& g,
-                               const void* src, void* dst, hipStream_t stream, int num_cus) {
+                               const void* src, void* dst, hipStream_t stream, int num_cus, const PlainTuning* tuning) {
     PlainArgs a;
     a.f.src = static_cast<const uint8_t*>(src);
     a.f.width = d.width;
@@ -346,14 +426,16 @@ hipError_t launch_encode_plain(const DeviceTables* d_tables, const astc_image_de
     a.nblocks = int(g.nblocks);
     a.dst = static_cast<uint32_t*>(dst);
     a.tables = d_tables;
+    PlainTuning t = tuning ? *tuning : default_plain_tuning(d, g.nblocks, num_cus);
+    if (d.block_dim != 6) t.lanes_per_block = 1;
+    if (t.blocks_per_lane == 0) t.blocks_per_lane = plain_blocks_per_lane(g.nblocks, d.block_dim, t.lanes_per_block, num_cus);
     const bool alpha = d.flags & FLAG_ALPHA;
-    const int bpl = plain_blocks_per_lane(g.nblocks, d.block_dim, num_cus);
     if (d.block_dim == 4) {
-        if (alpha) launch_plain_t<4, true>(a, g.nblocks, bpl, stream);
-        else launch_plain_t<4, false>(a, g.nblocks, bpl, stream);
+        if (alpha) launch_plain_t<4, true>(a, g.nblocks, t, stream);
+        else launch_plain_t<4, false>(a, g.nblocks, t, stream);
     } else {
-        if (alpha) launch_plain_t<6, true>(a, g.nblocks, bpl, stream);
-        else launch_plain_t<6, false>(a, g.nblocks, bpl, stream);
+        if (alpha) launch_plain_t<6, true>(a, g.nblocks, t, stream);
+        else launch_plain_t<6, false>(a, g.nblocks, t, stream);
     }
     return hipGetLastError();
 }

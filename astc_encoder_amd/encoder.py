@@ -14,7 +14,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from ._capi import (FLAG_ALPHA, FLAG_MAX_ACCUMULATION, FLAG_NORMAL, FLAG_SRGB, FMT_RG8, FMT_RGB8, FMT_RGBA8, ImageDesc,
-                    BYTES_PER_TEXEL, Request, ServerConfig, ServerStats, Tuning, check, lib)
+                    BYTES_PER_TEXEL, PlainTuning, Request, ServerConfig, ServerStats, Tuning, check, lib)
 
 _CHANNELS_TO_FMT = {4: FMT_RGBA8, 3: FMT_RGB8, 2: FMT_RG8}
 
@@ -65,11 +65,13 @@ def output_size(desc):
     return check(lib().astc_output_size(ctypes.byref(desc)))
 
 
-def _stream_handle(stream):
+def _torch_stream(stream):
     import torch
-    if stream is None:
-        stream = torch.cuda.current_stream()
-    return ctypes.c_void_p(stream.cuda_stream)
+    return torch.cuda.current_stream() if stream is None else stream
+
+
+def _stream_handle(stream):
+    return ctypes.c_void_p(_torch_stream(stream).cuda_stream)
 
 
 class Encoder:
@@ -100,48 +102,100 @@ class Encoder:
         check(lib().astc_encode(self._ctx, ctypes.byref(desc), ctypes.c_void_p(src_ptr),
                                 ctypes.c_void_p(dst_ptr), _stream_handle(stream)))
 
-    def encode(self, texture, block_dim=4, flags=0, out=None, stream=None):
-        """Encode a device texture (H, W, C) uint8 tensor, C in {4, 3, 2}.
-
-        Returns a (nblocks, 16) uint8 device tensor in row-major block order."""
+    def _texture_desc(self, texture, block_dim, flags):
         import torch
         if texture.dtype != torch.uint8 or texture.dim() != 3 or texture.shape[2] not in _CHANNELS_TO_FMT:
             raise ValueError("texture must be an (H, W, 4|3|2) uint8 tensor")
         if not texture.is_cuda:
             raise ValueError("texture must be in device memory")
         h, w, c = texture.shape
-        if texture.stride(2) != 1 or texture.stride(1) != c:
-            texture = texture.contiguous()
-        desc = make_desc(w, h, _CHANNELS_TO_FMT[c], block_dim, flags, row_pitch=texture.stride(0))
+        return make_desc(w, h, _CHANNELS_TO_FMT[c], block_dim, flags, row_pitch=texture.stride(0))
+
+    @staticmethod
+    def _dense_rows(texture, stream):
+        """The texture itself if its rows hold contiguous texels, else a dense copy
+        kept alive for the launch stream (the caller's reference ends before the
+        asynchronous kernel does)."""
+        c = texture.shape[2]
+        if texture.stride(2) == 1 and texture.stride(1) == c:
+            return texture
+        dense = texture.contiguous()
+        dense.record_stream(_torch_stream(stream))
+        return dense
+
+    def _output_for(self, desc, texture, out):
+        import torch
         n = output_size(desc)
         if out is None:
-            out = torch.empty((n // 16, 16), dtype=torch.uint8, device=texture.device)
-        elif out.numel() < n or not out.is_contiguous():
-            raise ValueError("out is too small or not contiguous")
-        self.encode_raw(desc, texture.data_ptr(), out.data_ptr(), stream)
+            return torch.empty((n // 16, 16), dtype=torch.uint8, device=texture.device)
+        if out.dtype != torch.uint8 or out.numel() < n or not out.is_contiguous() or out.device != texture.device:
+            raise ValueError("out must be a contiguous uint8 tensor of at least output_size bytes on the texture's device")
         return out
+
+    def encode(self, texture, block_dim=4, flags=0, out=None, stream=None, tuning=None):
+        """Encode a device texture (H, W, C) uint8 tensor, C in {4, 3, 2}, whose
+        rows hold contiguous texels (any row pitch).  tuning: None (the library's
+        default) or (lanes_per_block, blocks_per_lane, prefetch).
+
+        Returns a (nblocks, 16) uint8 device tensor in row-major block order."""
+        self._texture_desc(texture, block_dim, flags)
+        texture = self._dense_rows(texture, stream)
+        desc = self._texture_desc(texture, block_dim, flags)
+        out = self._output_for(desc, texture, out)
+        if tuning is None:
+            self.encode_raw(desc, texture.data_ptr(), out.data_ptr(), stream)
+        else:
+            t = PlainTuning(int(tuning[0]), int(tuning[1]), int(tuning[2]), 0)
+            check(lib().astc_encode_tuned(self._ctx, ctypes.byref(desc), ctypes.c_void_p(texture.data_ptr()),
+                                          ctypes.c_void_p(out.data_ptr()), ctypes.byref(t), _stream_handle(stream)))
+        return out
+
+    def autotune_plain(self, texture, block_dim=4, flags=0, reps=10, stream=None):
+        """astc_plain_autotune on this texture: the fastest plain tuning as a dict."""
+        self._texture_desc(texture, block_dim, flags)
+        texture = self._dense_rows(texture, stream)
+        desc = self._texture_desc(texture, block_dim, flags)
+        out = self._output_for(desc, texture, None)
+        t = PlainTuning()
+        check(lib().astc_plain_autotune(self._ctx, ctypes.byref(desc), ctypes.c_void_p(texture.data_ptr()),
+                                        ctypes.c_void_p(out.data_ptr()), _stream_handle(stream), int(reps),
+                                        ctypes.byref(t)))
+        return t.as_dict()
 
 
     def decode(self, blocks, width, height, block_dim_x=4, block_dim_y=None, out=None, stream=None):
         """Decode (nblocks, 16) device blocks to an (H, W, 4) uint8 device tensor."""
         import torch
         block_dim_y = block_dim_x if block_dim_y is None else block_dim_y
+        nb = ((width + block_dim_x - 1) // block_dim_x) * ((height + block_dim_y - 1) // block_dim_y)
+        if blocks.dtype != torch.uint8 or not blocks.is_cuda or blocks.numel() < nb * 16:
+            raise ValueError(f"blocks must be a uint8 device tensor of at least {nb * 16} bytes")
+        if not blocks.is_contiguous():
+            blocks = blocks.contiguous()
+            blocks.record_stream(_torch_stream(stream))
         if out is None:
             out = torch.empty((height, width, 4), dtype=torch.uint8, device=blocks.device)
+        elif out.dtype != torch.uint8 or not out.is_contiguous() or out.numel() < height * width * 4:
+            raise ValueError("out must be a contiguous uint8 tensor of at least height * width * 4 bytes")
         check(lib().astc_decode(self._ctx, int(width), int(height), int(block_dim_x), int(block_dim_y),
-                                ctypes.c_void_p(blocks.contiguous().data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                ctypes.c_void_p(blocks.data_ptr()), ctypes.c_void_p(out.data_ptr()),
                                 _stream_handle(stream)))
         return out
 
     def downsample(self, level, stream=None):
         """Next mip level of a device (H, W, C) uint8 tensor (2x2 box filter)."""
         import torch
+        if level.dtype != torch.uint8 or level.dim() != 3 or level.shape[2] not in _CHANNELS_TO_FMT or not level.is_cuda:
+            raise ValueError("level must be an (H, W, 4|3|2) uint8 device tensor")
         h, w, c = level.shape
         desc = make_desc(w, h, _CHANNELS_TO_FMT[c], 4, 0)
         ow = w >> 1 if w >= 8 else w
         oh = h >> 1 if h >= 8 else h
+        if not level.is_contiguous():
+            level = level.contiguous()
+            level.record_stream(_torch_stream(stream))
         out = torch.empty((oh, ow, c), dtype=torch.uint8, device=level.device)
-        check(lib().astc_downsample(self._ctx, ctypes.byref(desc), ctypes.c_void_p(level.contiguous().data_ptr()),
+        check(lib().astc_downsample(self._ctx, ctypes.byref(desc), ctypes.c_void_p(level.data_ptr()),
                                     ctypes.c_void_p(out.data_ptr()), _stream_handle(stream)))
         return out
 
@@ -206,7 +260,12 @@ class Batch:
             self._keep.append(tex)
             descs[i] = make_desc(w, h, _CHANNELS_TO_FMT[c], dim, flags, row_pitch=tex.stride(0))
             nb = output_size(descs[i]) // 16
-            out = outputs[i] if outputs is not None else torch.empty((nb, 16), dtype=torch.uint8, device=tex.device)
+            if outputs is not None:
+                out = outputs[i]
+                if out.dtype != torch.uint8 or not out.is_contiguous() or out.numel() < nb * 16:
+                    raise ValueError(f"output {i} must be a contiguous uint8 tensor of at least {nb * 16} bytes")
+            else:
+                out = torch.empty((nb, 16), dtype=torch.uint8, device=tex.device)
             self.outputs.append(out)
             srcs[i] = tex.data_ptr()
             dsts[i] = out.data_ptr()

@@ -4,12 +4,15 @@ Default workload (one step = one plain per-texture launch over a resident
 texture): 4096x4096 RGBA8 sRGB, ASTC 6x6 with alpha, seeded value-noise content
 with rectangles and a cut-out alpha (466,489 blocks).  The line carries:
   value           Mtexels/s over real texels, inputs already in HBM
-  roofline        algorithmic HBM bytes per launch / average kernel duration
+  roofline_hbm    algorithmic HBM bytes per launch / average kernel duration
                   (HIP events on the launch stream) against 8 TB/s; `traffic`
                   is the measured HBM bytes per launch from rocprofv3 counters
                   (profiles/hbm_traffic_encode_plain_6x6.txt) when present
   roofline_valu   the same launch priced in canonical FP32 operations against
                   MI355X's 157.3 TFLOP/s vector peak
+  roofline        whichever of the two the kernel comes closest to
+  fp64_twin       binary32 vs binary64 runs of the same method on the three
+                  plain workloads (axis and endpoint error)
   cpu_baseline    the CPU oracle (scalar restatement of the reference encoder,
                   OpenMP over blocks) on this host's cores, on the same texture
   extras          the other plain workloads (and the batched / serving forms
@@ -66,7 +69,7 @@ def read_traffic(path):
     return float(m.group(1)) if m else None
 
 
-def time_plain(enc, wl, steps, warmup):
+def time_plain(enc, wl, steps, warmup, tuning=None):
     import ctypes
 
     import torch
@@ -78,11 +81,17 @@ def time_plain(enc, wl, steps, warmup):
     # one launch per step through the C ABI with everything precomputed
     desc = astc.make_desc(wl["w"], wl["h"], {4: astc.FMT_RGBA8, 3: astc.FMT_RGB8, 2: astc.FMT_RG8}[img.shape[2]],
                           wl["dim"], wl["flags"])
-    fn, ctx, dref = astc.lib().astc_encode, enc.handle, ctypes.byref(desc)
+    ctx, dref = enc.handle, ctypes.byref(desc)
     sp, dp, st = ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream.cuda_stream)
+    if tuning is None:
+        fn, args = astc.lib().astc_encode, (ctx, dref, sp, dp, st)
+    else:
+        from astc_encoder_amd._capi import PlainTuning
+        t = PlainTuning(*tuning, 0)
+        fn, args = astc.lib().astc_encode_tuned, (ctx, dref, sp, dp, ctypes.byref(t), st)
 
     def launch():
-        if fn(ctx, dref, sp, dp, st) != 0:
+        if fn(*args) != 0:
             raise astc.AstcError(-2, astc.lib().astc_last_error().decode())
 
     for _ in range(warmup):
@@ -287,9 +296,39 @@ def cpu_baseline(wl, img, gpu_blocks, min_s=10.0):
     # the oracle's own blocks through the oracle's spec decoder: the reference PSNR
     _, dec = orc.decode_image(ref, wl["w"], wl["h"], wl["dim"])
     return dict(value=round(wl["w"] * wl["h"] / med / 1e6, 2), unit="Mtexels/s", cores=cores, kind="port",
+                host_cpu=cpu_model(),
                 sample=f"whole texture ({wl['name']}), median of {len(times)} runs over {sum(times):.1f} s "
                        f"after 1 warm-up",
                 gpu_block_match=match, oracle_psnr_db=round(orc.psnr(img, wl["flags"], dec), 4))
+
+
+def fp64_twin(wls, threads):
+    """The oracle's binary32 run (the schedule the GPU runs) against a binary64 run
+    of the same method, over every block of each plain workload: principal-axis
+    error (unit axes, up to sign) and unrounded endpoint error in 8-bit units."""
+    from tests import oracle_ref as orc
+    out = {}
+    for key, wl in wls.items():
+        img = wl["gen"](0, wl["w"], wl["h"])
+        ax, ep, diff = orc.fp64_twin(img, wl["dim"], wl["flags"], threads=threads)
+        a = ax[~np.isnan(ax)]
+        out[key] = {"blocks": int(len(ep)), "axis_error_max": float(a.max()) if len(a) else 0.0,
+                    "axis_error_p999": float(np.percentile(a, 99.9)) if len(a) else 0.0,
+                    "axis_error_mean": float(a.mean()) if len(a) else 0.0,
+                    "blocks_axis_error_over_1e-3": int((a > 1e-3).sum()),
+                    "endpoint_error_max": float(ep.max()), "endpoint_error_p999": float(np.percentile(ep, 99.9)),
+                    "endpoint_error_mean": float(ep.mean()), "blocks_differing": round(float(diff.mean()), 6)}
+    return out
+
+
+def cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_small_case(wl, min_s=3.0):
@@ -312,6 +351,9 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="only the headline workload")
     ap.add_argument("--no-batched", action="store_true", help="skip the mixed-texture batched vs plain run")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving sweep")
+    ap.add_argument("--plain-tuning", default=None,
+                    help="lanes_per_block,blocks_per_lane,prefetch for the headline's plain launches "
+                         "(default: the library's choice)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run 3 device copies of 256 MiB (a known-size kernel for scaling HBM counters)")
     args = ap.parse_args()
@@ -333,12 +375,43 @@ def main():
         del a, c
     wls = workloads()
     head = wls["noise_rgba8_srgb_6x6"]
-    r = time_plain(enc, head, args.steps, args.warmup)
+    # The other GPU legs run first, so that the headline is timed with the GPU at
+    # its steady clock (a few hundred microseconds of launches after idle run
+    # slower while the clock rises).
+    extras = {}
+    if not args.no_extras:
+        for key in ("sinusoid_rgb8_4x4", "normal_rg8_4x4"):
+            wl = wls[key]
+            e = time_plain(enc, wl, max(10, args.steps), args.warmup)
+            extras[key] = {"mtexels_s": round(e["mtexels_s"], 1), "kernel_ms": round(e["kernel_ms"], 4),
+                           "psnr_db": e["psnr_db"],
+                           "valu_tflops": round(FLOPS_PER_BLOCK[wl["flops_key"]] * e["nblocks"] /
+                                                (e["kernel_ms"] * 1e-3) / 1e12, 2)}
+        extras["host_resident_headline"] = time_host_resident(enc, head, 20)
+        if not args.no_batched:
+            extras["mixed_textures_batched_vs_plain"] = batched_vs_plain(enc, max(5, args.steps // 5), 2)
+        if not args.no_serving:
+            extras["serving"] = serving_sweep(enc)
+    head_tuning = tuple(int(v) for v in args.plain_tuning.split(",")) if args.plain_tuning else None
+    r = time_plain(enc, head, args.steps, args.warmup, head_tuning)
     algo_bytes = r["texels"] * head["bpp"] + r["nblocks"] * 16
     achieved_gbs = algo_bytes / (r["kernel_ms"] * 1e-3) / 1e9
     flops = FLOPS_PER_BLOCK[head["flops_key"]] * r["nblocks"]
     achieved_tf = flops / (r["kernel_ms"] * 1e-3) / 1e12
     traffic = read_traffic(os.path.join(ROOT, "profiles", "hbm_traffic_encode_plain_6x6.txt"))
+    hbm = {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic}
+    valu = {"bound": "valu", "achieved": round(achieved_tf, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved_tf / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
+            "flops_per_block": FLOPS_PER_BLOCK[head["flops_key"]]}
+    # the roofline that binds is the one the kernel comes closest to
+    bound = dict(hbm if hbm["frac"] >= valu["frac"] else valu)
+    kernel = ("encode_plain_pair_kernel<true, true, 4>" if head_tuning and head_tuning[0] == 2
+              else "encode_plain_kernel<6, true, true, 4, 0>")
+    bound.update({"kernel": kernel, "plain_tuning": head_tuning or "library default",
+                  "kernel_ms": round(r["kernel_ms"], 4),
+                  "step_event_ms": round(r["step_event_ms"], 4), "algorithmic_bytes": algo_bytes,
+                  "algorithmic_flops": flops})
 
     line = {
         "metric": METRIC,
@@ -356,30 +429,12 @@ def main():
         "config": {"workload": head["name"] + ", plain per-texture launch", "width": head["w"],
                    "height": head["h"], "block": "6x6", "flags": "alpha|srgb", "blocks": r["nblocks"],
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "encode_plain_kernel<6, true, true, 4, 0>", "kernel_ms": round(r["kernel_ms"], 4),
-                     "step_event_ms": round(r["step_event_ms"], 4),
-                     "algorithmic_bytes": algo_bytes},
-        "roofline_valu": {"achieved": round(achieved_tf, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                          "frac": round(achieved_tf / FP32_PEAK_TFLOPS, 4), "flops_per_block":
-                          FLOPS_PER_BLOCK[head["flops_key"]]},
+        "roofline": bound,
+        "roofline_hbm": hbm,
+        "roofline_valu": valu,
     }
     line["psnr_db"] = r["psnr_db"]
     if not args.no_extras:
-        extras = {}
-        for key in ("sinusoid_rgb8_4x4", "normal_rg8_4x4"):
-            wl = wls[key]
-            e = time_plain(enc, wl, max(10, args.steps), args.warmup)
-            extras[key] = {"mtexels_s": round(e["mtexels_s"], 1), "kernel_ms": round(e["kernel_ms"], 4),
-                           "psnr_db": e["psnr_db"],
-                           "valu_tflops": round(FLOPS_PER_BLOCK[wl["flops_key"]] * e["nblocks"] /
-                                                (e["kernel_ms"] * 1e-3) / 1e12, 2)}
-        extras["host_resident_headline"] = time_host_resident(enc, head, 20)
-        if not args.no_batched:
-            extras["mixed_textures_batched_vs_plain"] = batched_vs_plain(enc, max(5, args.steps // 5), 2)
-        if not args.no_serving:
-            extras["serving"] = serving_sweep(enc)
         line["extras"] = extras
     # the CPU leg last: its 16 busy host threads stay away from the serving loop's timing
     if not args.no_cpu:
@@ -387,6 +442,7 @@ def main():
         small = wls["sinusoid_rgb8_4x4"]  # the reference's CPU-sized case: 1024^2 RGB8 4x4
         line["cpu_baseline"]["small_case"] = cpu_small_case(small)
         line["psnr_delta_vs_oracle_db"] = round(r["psnr_db"] - line["cpu_baseline"].pop("oracle_psnr_db"), 6)
+        line["fp64_twin"] = fp64_twin(wls, line["cpu_baseline"]["cores"])
     enc.close()
     print(json.dumps(line), flush=True)
 

@@ -95,6 +95,29 @@ int64_t astc_output_size(const astc_image_desc* desc);
  * One kernel launch on `stream`. */
 int astc_encode(astc_ctx* ctx, const astc_image_desc* desc, const void* src_dev, void* dst_dev, void* stream);
 
+/* The plain kernel's tuning.  lanes_per_block: 1 (one lane encodes a block)
+ * or 2 (two lanes share a 6x6 block, each holding three of its six texel rows;
+ * other block sizes and the max-accumulation search run 1).  blocks_per_lane:
+ * 1, 2, 4 or 8 (a workgroup covers 256 / lanes_per_block x this many blocks),
+ * or 0 for the library's choice.  prefetch: 1 issues the next block's loads
+ * before the current block is encoded.  Every tuning gives the same blocks. */
+typedef struct astc_plain_tuning {
+    int32_t lanes_per_block;
+    int32_t blocks_per_lane;
+    int32_t prefetch;
+    int32_t reserved;
+} astc_plain_tuning;
+
+/* astc_encode with an explicit plain-kernel tuning (astc_encode uses the
+ * library's default for the image).  Same blocks as astc_encode. */
+int astc_encode_tuned(astc_ctx* ctx, const astc_image_desc* desc, const void* src_dev, void* dst_dev,
+                      const astc_plain_tuning* tuning, void* stream);
+/* Time every plain tuning on this image (reps launches each after one warm-up,
+ * on `stream`; synchronous; dst_dev is overwritten) and write the fastest to
+ * *best.  No reference counterpart (the reference has one fixed dispatch). */
+int astc_plain_autotune(astc_ctx* ctx, const astc_image_desc* desc, const void* src_dev, void* dst_dev, void* stream,
+                        int reps, astc_plain_tuning* best);
+
 /* Decode blocks (row-major, ceil(W/bx) x ceil(H/by) of them, 16-byte aligned)
  * to a width x height RGBA8 image (4-byte aligned, row pitch width * 4).
  * Single-partition LDR ASTC: block sizes 4..12, any weight grid and

@@ -348,24 +348,37 @@ struct Arith {
         if (canon) return std::fma(a[3], b[3], std::fma(a[2], b[2], std::fma(a[1], b[1], a[0] * b[0])));
         return ((a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]) + a[3] * b[3];
     }
-    /* Sum over the block's texels of f(k), left to right over k (both schedules;
-     * ASTC_Encode.hlsl:143-146). */
+    /* Sum over the block's texels of f(k), left to right over k
+     * (ASTC_Encode.hlsl:143-146).  Canonical 6x6: the two halves of the block,
+     * rows 0-2 (texels 0-17) and rows 3-5 (texels 18-35), are each summed left
+     * to right and the two sums added, so that each half can be summed by its
+     * own lane and the halves combined with one exchange. */
     template <typename F> T block_sum(int dim, F f) const {
+        const int n = dim * dim, h = (canon && dim == 6) ? n / 2 : n;
         T s = f(0);
-        for (int k = 1; k < dim * dim; ++k) s = s + f(k);
-        return s;
+        for (int k = 1; k < h; ++k) s = s + f(k);
+        if (h == n) return s;
+        T s2 = f(h);
+        for (int k = h + 1; k < n; ++k) s2 = s2 + f(k);
+        return s + s2;
     }
     /* Sum over texels of a[k]*b[k]: canonical is one fma chain starting with a
-     * product; source adds separately rounded products (ASTC_Encode.hlsl:151-161). */
+     * product (for 6x6, one per half as in block_sum, then added); source adds
+     * separately rounded products (ASTC_Encode.hlsl:151-161). */
     T block_dot(int dim, const T* a, const T* b, int stride) const {
+        const int n = dim * dim;
         if (!canon) {
             T s = 0;
-            for (int k = 0; k < dim * dim; ++k) s = s + a[k * stride] * b[k * stride];
+            for (int k = 0; k < n; ++k) s = s + a[k * stride] * b[k * stride];
             return s;
         }
+        const int h = dim == 6 ? n / 2 : n;
         T r = a[0] * b[0];
-        for (int k = 1; k < dim * dim; ++k) r = std::fma(a[k * stride], b[k * stride], r);
-        return r;
+        for (int k = 1; k < h; ++k) r = std::fma(a[k * stride], b[k * stride], r);
+        if (h == n) return r;
+        T r2 = a[h * stride] * b[h * stride];
+        for (int k = h + 1; k < n; ++k) r2 = std::fma(a[k * stride], b[k * stride], r2);
+        return r + r2;
     }
     /* x / n for the constant divisors of :147 and :162.  FXC turns a division by
      * a literal into a multiplication by its reciprocal; the canonical schedule
@@ -998,6 +1011,47 @@ int64_t oracle_encode_image(const uint8_t* src, int width, int height, int row_p
         encode_block_impl<float>(tex, dim, alpha, sched, out_blocks + 16 * id, nullptr, method);
     }
     return nblk;
+}
+
+void oracle_fp64_twin(const uint8_t* src, int width, int height, int row_pitch, int src_format, int dim, int flags,
+                      int threads, double* axis_err, double* endpoint_err, uint8_t* differ) {
+    /* The same method run in binary32 (canonical schedule, what the GPU runs)
+     * and in binary64 on every block of an image (ASTC_Encode.hlsl:510-550):
+     * axis error = |a32 - a64| up to sign (-1 where the binary64 axis is zero);
+     * endpoint error = largest |e32 - e64| over the unrounded endpoint
+     * channels, in 8-bit units; differ = the two blocks' bytes differ. */
+    const int xb = (width + dim - 1) / dim, yb = (height + dim - 1) / dim;
+    const int64_t nblk = (int64_t)xb * yb;
+    const bool alpha = flags & ORACLE_FLAG_ALPHA;
+    const int method = (flags & ORACLE_FLAG_MAX_ACCUMULATION) ? ORACLE_METHOD_MAX_ACCUMULATION : ORACLE_METHOD_PCA;
+    (void)tables(); (void)taps6(); (void)srgb_lut(); (void)unorm_lut();
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t id = 0; id < nblk; ++id) {
+        float t32[36 * 4];
+        double t64[36 * 4];
+        const int by = (int)(id / xb), bx = (int)(id - (int64_t)by * xb);
+        fetch_block<float>(src, width, height, row_pitch, src_format, dim, flags, bx, by, t32);
+        fetch_block<double>(src, width, height, row_pitch, src_format, dim, flags, bx, by, t64);
+        uint8_t b32[16], b64[16];
+        oracle_block_trace r32, r64;
+        encode_block_impl<float>(t32, dim, alpha, ORACLE_SCHED_CANONICAL, b32, &r32, method);
+        encode_block_impl<double>(t64, dim, alpha, ORACLE_SCHED_CANONICAL, b64, &r64, method);
+        double nrm = 0.0, dm = 0.0, dp = 0.0, ep = 0.0;
+        for (int c = 0; c < 4; ++c) {
+            nrm += r64.axis[c] * r64.axis[c];
+            dm += (r32.axis[c] - r64.axis[c]) * (r32.axis[c] - r64.axis[c]);
+            dp += (r32.axis[c] + r64.axis[c]) * (r32.axis[c] + r64.axis[c]);
+            ep = std::max(ep, std::max(std::fabs(r32.e0[c] - r64.e0[c]), std::fabs(r32.e1[c] - r64.e1[c])));
+        }
+        axis_err[id] = nrm > 0.0 ? std::sqrt(std::min(dm, dp)) : -1.0;
+        endpoint_err[id] = ep;
+        differ[id] = std::memcmp(b32, b64, 16) != 0;
+    }
 }
 
 void oracle_srgb_table(float out[256]) { std::memcpy(out, srgb_lut().f, sizeof(float) * 256); }

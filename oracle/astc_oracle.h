@@ -69,6 +69,13 @@ int oracle_encode_block_method_f64(const double* texels, int dim, int has_alpha,
  * out-of-bounds texels are zero, normal maps force b = a = 1, then x255.
  * UNORM channels convert as binary32 c * (1/255) (the hardware conversion)
  * before the x255; sRGB colour channels through oracle_srgb_table. */
+/* Binary32 against binary64 runs of the same method over every block of an
+ * image, per block: axis error (unit axes compared up to sign; -1 where the
+ * binary64 axis is zero), endpoint error (largest difference of the unrounded
+ * endpoint channels, 8-bit units), and whether the blocks' bytes differ. */
+void oracle_fp64_twin(const uint8_t* src, int width, int height, int row_pitch, int src_format, int dim, int flags,
+                      int threads, double* axis_err, double* endpoint_err, uint8_t* differ);
+
 void oracle_fetch_block_f32(const uint8_t* src, int width, int height, int row_pitch, int src_format,
                             int dim, int flags, int bx, int by, float* texels);
 void oracle_fetch_block_f64(const uint8_t* src, int width, int height, int row_pitch, int src_format,

@@ -48,6 +48,7 @@ def lib():
             "oracle_fetch_block_f32": (None, [P, I, I, I, I, I, I, I, I, P]),
             "oracle_fetch_block_f64": (None, [P, I, I, I, I, I, I, I, I, P]),
             "oracle_encode_image": (ctypes.c_int64, [P, I, I, I, I, I, I, I, I, P]),
+            "oracle_fp64_twin": (None, [P, I, I, I, I, I, I, I, P, P, P]),
             "oracle_srgb_table": (None, [P]),
             "oracle_unorm_table": (None, [P]),
             "oracle_tables": (None, [P, P, P, P]),
@@ -86,6 +87,19 @@ def oracle_encode(img, dim, flags, sched=SCHED_CANONICAL, threads=0):
     rc = lib().oracle_encode_image(_ptr(img), w, h, w * c, FMT[c], dim, flags, sched, threads, _ptr(out))
     assert rc == n
     return out
+
+
+def fp64_twin(img, dim, flags, threads=0):
+    """Binary32 vs binary64 runs of the same method over every block of an image:
+    per-block axis error (NaN where the binary64 axis is zero), endpoint error
+    (8-bit units) and byte difference."""
+    img = np.ascontiguousarray(img)
+    h, w, c = img.shape
+    n = ((w + dim - 1) // dim) * ((h + dim - 1) // dim)
+    ax, ep, diff = np.zeros(n), np.zeros(n), np.zeros(n, np.uint8)
+    lib().oracle_fp64_twin(_ptr(img), w, h, w * c, FMT[c], dim, flags, threads, _ptr(ax), _ptr(ep), _ptr(diff))
+    ax[ax < 0] = np.nan
+    return ax, ep, diff.astype(bool)
 
 
 def encode_block(texels, dim, has_alpha, sched=SCHED_CANONICAL, f64=False, method=METHOD_PCA):

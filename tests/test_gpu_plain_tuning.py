@@ -1,0 +1,126 @@
+"""Plain-kernel tunings (astc_encode_tuned, astc_plain_autotune) against the oracle.
+
+Every tuning must give the oracle's blocks bit for bit: one or two lanes per
+6x6 block (the two-lane encoder sums, projects and samples half a block per
+lane and combines the halves by lane exchange), blocks per lane, and next-block
+prefetch, over every format, flag and channel class, ragged and tiled sizes,
+unaligned sources with padded rows, the largest dimensions, seeded random cases
+and the full-size 4096x4096 RGBA8 sRGB texture.
+"""
+import numpy as np
+import pytest
+
+import astc_encoder_amd as astc
+from astc_encoder_amd import synth
+from tests import oracle_ref as orc
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+TUNINGS = [(2, 0, 0), (2, 0, 1), (2, 1, 1), (2, 8, 0), (1, 0, 0), (1, 0, 1), (1, 2, 0)]
+
+
+@pytest.fixture(scope="module")
+def enc():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    e = astc.Encoder(0)
+    yield e
+    e.close()
+
+
+def tuned(enc, tex, dim, flags, tuning):
+    out = enc.encode(tex, block_dim=dim, flags=flags, tuning=tuning)
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+def assert_same(got, want, what):
+    if not np.array_equal(got, want):
+        bad = np.nonzero(~np.all(got == want, axis=1))[0]
+        raise AssertionError(f"{what}: {len(bad)} of {len(want)} blocks differ, first {bad[:5].tolist()}")
+
+
+@pytest.mark.parametrize("channels", [4, 3, 2])
+@pytest.mark.parametrize("flags", range(8))
+@pytest.mark.parametrize("shape", [(45, 38), (36, 48)])
+def test_six_by_six_every_format_flag_and_class(enc, channels, flags, shape):
+    rng = np.random.default_rng(channels * 100 + flags * 10 + shape[0])
+    img = rng.integers(0, 256, size=shape + (channels,), dtype=np.uint8)
+    # two-level tiles as well as noise: flat and near-flat blocks take the short-axis paths
+    img[: shape[0] // 2, : shape[1] // 3] = img[0, 0]
+    want = orc.oracle_encode(img, 6, flags)
+    tex = torch.from_numpy(img).cuda()
+    for t in TUNINGS:
+        assert_same(tuned(enc, tex, 6, flags, t), want, f"{channels}ch flags {flags} {shape} tuning {t}")
+
+
+@pytest.mark.parametrize("channels", [4, 3, 2])
+def test_unaligned_source_and_padded_pitch(enc, channels):
+    rng = np.random.default_rng(90 + channels)
+    raw = torch.from_numpy(rng.integers(0, 256, size=(53, 47 + 3, channels), dtype=np.uint8)).cuda()
+    flat = raw.view(-1)[1:1 + 53 * 49 * channels].view(53, 49, channels)  # source 1 byte past a word
+    for tex in (raw[:, :47], flat[:, :46]):
+        want = orc.oracle_encode(tex.cpu().numpy(), 6, astc.FLAG_ALPHA)
+        for t in TUNINGS:
+            assert_same(tuned(enc, tex, 6, astc.FLAG_ALPHA, t), want, f"{channels}ch tuning {t}")
+
+
+@pytest.mark.parametrize("shape", [(1, 16777215), (16777215, 1)])
+def test_largest_dimensions_two_lanes(enc, shape):
+    rng = np.random.default_rng(shape[0] % 97)
+    img = rng.integers(0, 256, size=shape + (4,), dtype=np.uint8)
+    want = orc.oracle_encode(img, 6, astc.FLAG_ALPHA | astc.FLAG_SRGB, threads=16)
+    assert_same(tuned(enc, torch.from_numpy(img).cuda(), 6, astc.FLAG_ALPHA | astc.FLAG_SRGB, (2, 0, 1)), want,
+                f"{shape}")
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_random_cases_random_tunings(enc, seed):
+    rng = np.random.default_rng(7000 + seed)
+    w, h = int(rng.integers(1, 400)), int(rng.integers(1, 400))
+    c = int(rng.choice([4, 3, 2]))
+    flags = int(rng.integers(0, 8))
+    kind = int(rng.integers(0, 3))
+    img = rng.integers(0, 256, size=(h, w, c), dtype=np.uint8)
+    if kind == 1:
+        img[:] = img[0, 0]
+    elif kind == 2:
+        img = np.where(rng.random((h, w, 1)) < 0.5, img[0, 0], img[-1, -1]).astype(np.uint8)
+    t = (int(rng.choice([1, 2])), int(rng.choice([0, 1, 2, 4, 8])), int(rng.integers(0, 2)))
+    print(f"case seed={seed} {w}x{h}x{c} flags={flags} kind={kind} tuning={t}", flush=True)
+    assert_same(tuned(enc, torch.from_numpy(img).cuda(), 6, flags, t), orc.oracle_encode(img, 6, flags),
+                f"seed {seed}")
+
+
+def test_full_size_texture_two_lanes(enc):
+    img = synth.noise_rgba8(0, 4096, 4096)
+    flags = astc.FLAG_ALPHA | astc.FLAG_SRGB
+    want = orc.oracle_encode(img, 6, flags, threads=16)
+    tex = torch.from_numpy(img).cuda()
+    for t in [(2, 0, 0), (2, 0, 1), (1, 0, 0)]:
+        assert_same(tuned(enc, tex, 6, flags, t), want, f"4096x4096 tuning {t}")
+
+
+def test_autotune_picks_a_valid_tuning(enc):
+    img = synth.noise_rgba8(3, 512, 512)
+    flags = astc.FLAG_ALPHA | astc.FLAG_SRGB
+    tex = torch.from_numpy(img).cuda()
+    best = enc.autotune_plain(tex, 6, flags, reps=3)
+    assert best["lanes_per_block"] in (1, 2) and best["blocks_per_lane"] in (1, 2, 4, 8)
+    assert best["prefetch"] in (0, 1)
+    got = tuned(enc, tex, 6, flags, (best["lanes_per_block"], best["blocks_per_lane"], best["prefetch"]))
+    assert_same(got, orc.oracle_encode(img, 6, flags), "autotuned")
+    # 4x4 images and the max-accumulation search have one lane per block only
+    best4 = enc.autotune_plain(tex, 4, flags, reps=1)
+    assert best4["lanes_per_block"] == 1
+    bestm = enc.autotune_plain(tex, 6, flags | astc.FLAG_MAX_ACCUMULATION, reps=1)
+    assert bestm["lanes_per_block"] == 1
+
+
+@pytest.mark.parametrize("tuning", [(3, 0, 0), (0, 0, 0), (2, 3, 0), (2, 16, 0), (1, 0, 2), (2, -1, 0)])
+def test_out_of_range_tunings_are_rejected(enc, tuning):
+    tex = torch.zeros((12, 12, 4), dtype=torch.uint8, device="cuda")
+    with pytest.raises(astc.AstcError, match="plain tuning out of range"):
+        enc.encode(tex, block_dim=6, flags=0, tuning=tuning)
