@@ -117,6 +117,11 @@ SIGNATURES = {
     "astc_server_run_trace": (I32, [P, ctypes.POINTER(Request), I32, ctypes.POINTER(ctypes.c_double),
                                     ctypes.POINTER(ServerStats)]),
     "astc_server_destroy": (I32, [P]),
+    "astc_server_reserve": (I32, [P, PDESC, I32, ctypes.POINTER(P), ctypes.POINTER(I64)]),
+    "astc_server_submit": (I32, [P, I64, P]),
+    "astc_server_poll": (I32, [P, ctypes.POINTER(I64), I32, ctypes.POINTER(I32)]),
+    "astc_server_blocks": (I32, [P, I64, ctypes.POINTER(P), ctypes.POINTER(I64)]),
+    "astc_server_release": (I32, [P, I64]),
 }
 
 _lib = None

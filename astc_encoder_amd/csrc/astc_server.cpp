@@ -1,20 +1,27 @@
 // astc_server.cpp -- continuous batching of compression requests
 // (include/astc_mi355x.h, astc_server_*).
 //
-// One host thread runs the loop: poll the in-flight batches' events (complete
-// their requests, free their ring space), release requests whose arrival time
-// has passed, and whenever a batch slot is free form a batch from the waiting
-// requests that fit in the ring.  A batch is: one device-to-device copy of each
-// request's sources into the ring, one host-to-device copy of the batch's work
-// list into the slot's buffers, one replay of the slot's captured graph
-// (counter reset + persistent kernel), optional copies of the blocks to the
-// requests' destinations, and an event.  Slots use separate streams, so a batch
-// formed while others run overlaps with them on the device.
+// The server owns a device ring (the HBM budget) and up to max_inflight batch
+// slots, each with its own stream and captured graph of the persistent kernel.
+// A batch is: waits for the requests' sources (an event per request submitted
+// from a caller's stream), device-to-device copies of sources into the ring
+// for requests that were handed over elsewhere, one host-to-device copy of the
+// batch's work list into the slot's buffers, one replay of the slot's graph
+// (counter reset + persistent kernel), optional copies of the blocks out, and
+// an event.  Slots use separate streams, so a batch formed while others run
+// overlaps with them on the device.
+//
+// Two ways to feed it: astc_server_run_trace replays a trace of requests whose
+// sources lie elsewhere in device memory (copied into the ring on admission),
+// in real time on the calling thread; the online calls (reserve, submit, poll,
+// blocks, release) hand the caller ring space to write its sources into, so
+// admission needs no copy, and leave the blocks in the ring until released.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <deque>
 #include <new>
+#include <unordered_map>
 #include <thread>
 #include <vector>
 
@@ -74,6 +81,8 @@ class Ring {
         if (live_.empty()) head_ = 0;
     }
     int64_t peak() const { return peak_; }
+    bool empty() const { return live_.empty(); }
+    void reset_peak() { peak_ = used_; }
 
   private:
     struct A { int64_t off, bytes; bool freed; };
@@ -96,8 +105,9 @@ struct Slot {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool busy = false;
-    std::vector<int> reqs;
-    std::vector<int64_t> offs;
+    std::vector<int> reqs;        // run_trace: indices of the batch's requests
+    std::vector<int64_t> offs;    // run_trace: their ring offsets
+    std::vector<int64_t> tickets; // online: the batch's tickets
 };
 
 struct ReqGeom {
@@ -117,12 +127,47 @@ struct ReqGeom {
 
 }  // namespace
 
+namespace {
+// An online request: its geometry, ring space, state and the event after which
+// its sources are in place (NULL: in place when submitted).
+struct Online {
+    ReqGeom geo;
+    int64_t off = 0;
+    enum State { RESERVED, SUBMITTED, IN_FLIGHT, DONE } state = RESERVED;
+    hipEvent_t ready = nullptr;
+};
+
+// One request of a batch being formed: its geometry and ring offset, the event
+// to wait for, a source to copy into the ring first and a destination to copy
+// the blocks to afterwards (each optional).
+struct BatchEntry {
+    const ReqGeom* geo;
+    int64_t off;
+    hipEvent_t ready;
+    const void* copy_src;
+    void* copy_dst;
+};
+}  // namespace
+
 struct astc_server {
     astc_ctx* ctx = nullptr;
     astc_server_config cfg{};
     int num_cus = 0;
     uint8_t* ring_base = nullptr;
     std::vector<Slot> slots;
+    Ring ring{0};
+    // online requests
+    std::unordered_map<int64_t, Online> online;
+    std::deque<int64_t> submitted;
+    std::deque<int64_t> completed;  // completed tickets not yet reported by poll
+    int64_t next_ticket = 1;
+    std::vector<hipEvent_t> free_events;
+    // batch formation scratch
+    std::vector<BatchLevel> levels;
+    std::vector<int> modes;
+    std::vector<BatchSeg> segs;
+    std::vector<BatchItem> items;
+    std::vector<BatchEntry> entries;
 };
 
 namespace {
@@ -226,6 +271,96 @@ int geometry(const astc_request& r, ReqGeom* g) {
     return ASTC_OK;
 }
 
+int64_t ring_bytes(const ReqGeom& g) { return g.src_bytes + align_up(g.out_bytes); }
+
+// Launch one batch of sv->entries on slot s (idle).  On success the slot is
+// busy until its `done` event completes.
+int launch_batch(astc_server* sv, Slot& s) {
+    std::vector<BatchLevel>& levels = sv->levels;
+    std::vector<int>& modes = sv->modes;
+    levels.clear();
+    modes.clear();
+    for (const BatchEntry& en : sv->entries) {
+        uint8_t* base = sv->ring_base + en.off;
+        for (size_t i = 0; i < en.geo->levels.size(); ++i) {
+            BatchLevel L = en.geo->levels[i];
+            L.f.src = base + reinterpret_cast<uintptr_t>(L.f.src);
+            L.dst = reinterpret_cast<uint32_t*>(base + en.geo->src_bytes + reinterpret_cast<uintptr_t>(L.dst));
+            levels.push_back(L);
+            modes.push_back(en.geo->modes[i]);
+        }
+    }
+    int first_light = 0;
+    plan_work(levels, modes, sv->cfg.tuning.blocks_per_lane, &sv->segs, &sv->items, &first_light);
+    int rc = ASTC_OK;
+    if (levels.size() > s.cap_levels || sv->segs.size() > s.cap_segs || sv->items.size() > s.cap_items) {
+        // grow every idle slot at once: regrowing (allocation + graph capture)
+        // costs milliseconds, so it should not recur slot by slot mid-stream
+        for (Slot& o : sv->slots)
+            if (!o.busy && rc == ASTC_OK)
+                rc = ensure_capacity(sv, o, std::max(levels.size(), o.cap_levels),
+                                     std::max(sv->segs.size(), o.cap_segs), std::max(sv->items.size(), o.cap_items));
+        if (rc != ASTC_OK) return rc;
+    }
+    std::memcpy(s.h_levels, levels.data(), levels.size() * sizeof(BatchLevel));
+    std::memcpy(s.h_segs, sv->segs.data(), sv->segs.size() * sizeof(BatchSeg));
+    std::memcpy(s.h_items, sv->items.data(), sv->items.size() * sizeof(BatchItem));
+    s.h_scalars[0] = (int)sv->items.size();
+    s.h_scalars[1] = first_light;
+    s.h_scalars[2] = 0;
+    s.h_scalars[3] = 0;
+    hipError_t e = hipSuccess;
+    hipEvent_t last_wait = nullptr;
+    for (const BatchEntry& en : sv->entries) {
+        if (e == hipSuccess && en.ready && en.ready != last_wait) {
+            e = hipStreamWaitEvent(s.stream, en.ready, 0);
+            last_wait = en.ready;
+        }
+        if (e == hipSuccess && en.copy_src)
+            e = hipMemcpyAsync(sv->ring_base + en.off, en.copy_src, en.geo->src_copy, hipMemcpyDeviceToDevice, s.stream);
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(s.d_levels, s.h_levels, levels.size() * sizeof(BatchLevel), hipMemcpyHostToDevice, s.stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(s.d_segs, s.h_segs, sv->segs.size() * sizeof(BatchSeg), hipMemcpyHostToDevice, s.stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(s.d_items, s.h_items, sv->items.size() * sizeof(BatchItem), hipMemcpyHostToDevice,
+                           s.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s.d_scalars, s.h_scalars, 4 * sizeof(int), hipMemcpyHostToDevice, s.stream);
+    if (e == hipSuccess) e = hipGraphLaunch(s.exec, s.stream);
+    for (const BatchEntry& en : sv->entries) {
+        const ReqGeom& q = *en.geo;
+        uint8_t* base = sv->ring_base + en.off;
+        for (size_t j = 0; j < q.plain.size() && e == hipSuccess; ++j) {
+            const BatchLevel& L = q.levels[q.plain[j]];
+            e = launch_encode_plain(sv->ctx->d_tables, q.plain_desc[j], q.plain_geom[j],
+                                    base + reinterpret_cast<uintptr_t>(L.f.src),
+                                    base + q.src_bytes + reinterpret_cast<uintptr_t>(L.dst), s.stream, sv->num_cus);
+        }
+    }
+    for (const BatchEntry& en : sv->entries)
+        if (e == hipSuccess && en.copy_dst)
+            e = hipMemcpyAsync(en.copy_dst, sv->ring_base + en.off + en.geo->src_bytes, en.geo->out_bytes,
+                               hipMemcpyDeviceToDevice, s.stream);
+    if (e == hipSuccess) e = hipEventRecord(s.done, s.stream);
+    if (e != hipSuccess) return hip_fail(e, "batch submission");
+    s.busy = true;
+    return ASTC_OK;
+}
+
+hipError_t take_event(astc_server* sv, hipEvent_t* ev) {
+    if (!sv->free_events.empty()) {
+        *ev = sv->free_events.back();
+        sv->free_events.pop_back();
+        return hipSuccess;
+    }
+    return hipEventCreateWithFlags(ev, hipEventDisableTiming);
+}
+
+void give_event(astc_server* sv, hipEvent_t ev) {
+    if (ev) sv->free_events.push_back(ev);
+}
+
 }  // namespace
 
 extern "C" {
@@ -266,6 +401,7 @@ int astc_server_create(astc_ctx* ctx, const astc_server_config* cfg, astc_server
             rc = hip_fail(e, "hipDeviceGetAttribute");
         else if ((e = hipMalloc(&sv->ring_base, cfg->hbm_budget_bytes)) != hipSuccess)
             rc = fail(ASTC_ERR_NO_MEMORY, std::string("ring allocation failed: ") + hipGetErrorString(e));
+        sv->ring = Ring(cfg->hbm_budget_bytes);
         if (rc == ASTC_OK) {
             sv->slots.resize(cfg->max_inflight);
             for (Slot& s : sv->slots) {
@@ -327,6 +463,9 @@ int astc_server_destroy(astc_server* sv) {
         if (s.side) (void)hipStreamDestroy(s.side);
         if (s.stream) (void)hipStreamDestroy(s.stream);
     }
+    for (auto& kv : sv->online)
+        if (kv.second.ready) (void)hipEventDestroy(kv.second.ready);
+    for (hipEvent_t ev : sv->free_events) (void)hipEventDestroy(ev);
     if (sv->ring_base) (void)hipFree(sv->ring_base);
     delete sv;
     return ASTC_OK;
@@ -352,14 +491,12 @@ int astc_server_run_trace(astc_server* sv, const astc_request* reqs, int n, doub
             st.blocks += geo[i].blocks;
         }
         st.requests = n;
-        Ring ring(sv->cfg.hbm_budget_bytes);
+        if (!sv->online.empty()) return fail(ASTC_ERR_INVALID_ARG, "online requests are outstanding on this server");
+        Ring& ring = sv->ring;
+        ring.reset_peak();
         std::deque<int> waiting;
         int next = 0, done = 0, inflight = 0;
         int64_t batched_reqs = 0;
-        std::vector<BatchLevel> levels;
-        std::vector<int> modes;
-        std::vector<BatchSeg> segs;
-        std::vector<BatchItem> items;
         using clk = std::chrono::steady_clock;
         const auto t0 = clk::now();
         auto now = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
@@ -397,84 +534,18 @@ int astc_server_run_trace(astc_server* sv, const astc_request* reqs, int n, doub
                 if (s.busy || waiting.empty()) continue;
                 s.reqs.clear();
                 s.offs.clear();
-                levels.clear();
-                modes.clear();
+                sv->entries.clear();
                 while (!waiting.empty() && (int)s.reqs.size() < sv->cfg.max_batch_requests) {
                     const int r = waiting.front();
                     int64_t off;
-                    if (!ring.alloc(geo[r].src_bytes + align_up(geo[r].out_bytes), &off)) break;
+                    if (!ring.alloc(ring_bytes(geo[r]), &off)) break;
                     waiting.pop_front();
                     s.reqs.push_back(r);
                     s.offs.push_back(off);
-                    uint8_t* base = sv->ring_base + off;
-                    for (size_t i = 0; i < geo[r].levels.size(); ++i) {
-                        BatchLevel L = geo[r].levels[i];
-                        L.f.src = base + reinterpret_cast<uintptr_t>(L.f.src);
-                        L.dst = reinterpret_cast<uint32_t*>(base + geo[r].src_bytes + reinterpret_cast<uintptr_t>(L.dst));
-                        levels.push_back(L);
-                        modes.push_back(geo[r].modes[i]);
-                    }
+                    sv->entries.push_back(BatchEntry{&geo[r], off, nullptr, reqs[r].src_dev, reqs[r].dst_dev});
                 }
                 if (s.reqs.empty()) break;  // the ring is full: wait for completions
-                int first_light = 0;
-                plan_work(levels, modes, sv->cfg.tuning.blocks_per_lane, &segs, &items, &first_light);
-                if (levels.size() > s.cap_levels || segs.size() > s.cap_segs || items.size() > s.cap_items) {
-                    // grow every idle slot at once: regrowing (allocation + graph capture)
-                    // costs milliseconds, so it should not recur slot by slot mid-stream
-                    for (Slot& o : sv->slots)
-                        if (!o.busy && rc == ASTC_OK)
-                            rc = ensure_capacity(sv, o, std::max(levels.size(), o.cap_levels),
-                                                 std::max(segs.size(), o.cap_segs), std::max(items.size(), o.cap_items));
-                    if (rc != ASTC_OK) break;
-                }
-                std::memcpy(s.h_levels, levels.data(), levels.size() * sizeof(BatchLevel));
-                std::memcpy(s.h_segs, segs.data(), segs.size() * sizeof(BatchSeg));
-                std::memcpy(s.h_items, items.data(), items.size() * sizeof(BatchItem));
-                s.h_scalars[0] = (int)items.size();
-                s.h_scalars[1] = first_light;
-                s.h_scalars[2] = 0;
-                s.h_scalars[3] = 0;
-                e = hipSuccess;
-                for (size_t k = 0; k < s.reqs.size() && e == hipSuccess; ++k) {
-                    const int r = s.reqs[k];
-                    e = hipMemcpyAsync(sv->ring_base + s.offs[k], reqs[r].src_dev, geo[r].src_copy,
-                                       hipMemcpyDeviceToDevice, s.stream);
-                }
-                if (e == hipSuccess)
-                    e = hipMemcpyAsync(s.d_levels, s.h_levels, levels.size() * sizeof(BatchLevel), hipMemcpyHostToDevice,
-                                       s.stream);
-                if (e == hipSuccess)
-                    e = hipMemcpyAsync(s.d_segs, s.h_segs, segs.size() * sizeof(BatchSeg), hipMemcpyHostToDevice,
-                                       s.stream);
-                if (e == hipSuccess)
-                    e = hipMemcpyAsync(s.d_items, s.h_items, items.size() * sizeof(BatchItem), hipMemcpyHostToDevice,
-                                       s.stream);
-                if (e == hipSuccess)
-                    e = hipMemcpyAsync(s.d_scalars, s.h_scalars, 4 * sizeof(int), hipMemcpyHostToDevice, s.stream);
-                if (e == hipSuccess) e = hipGraphLaunch(s.exec, s.stream);
-                for (size_t k = 0; k < s.reqs.size() && e == hipSuccess; ++k) {
-                    const ReqGeom& q = geo[s.reqs[k]];
-                    uint8_t* base = sv->ring_base + s.offs[k];
-                    for (size_t j = 0; j < q.plain.size() && e == hipSuccess; ++j) {
-                        const BatchLevel& L = q.levels[q.plain[j]];
-                        e = launch_encode_plain(sv->ctx->d_tables, q.plain_desc[j], q.plain_geom[j],
-                                                base + reinterpret_cast<uintptr_t>(L.f.src),
-                                                base + q.src_bytes + reinterpret_cast<uintptr_t>(L.dst), s.stream,
-                                                sv->num_cus);
-                    }
-                }
-                for (size_t k = 0; k < s.reqs.size() && e == hipSuccess; ++k) {
-                    const int r = s.reqs[k];
-                    if (reqs[r].dst_dev)
-                        e = hipMemcpyAsync(reqs[r].dst_dev, sv->ring_base + s.offs[k] + geo[r].src_bytes,
-                                           geo[r].out_bytes, hipMemcpyDeviceToDevice, s.stream);
-                }
-                if (e == hipSuccess) e = hipEventRecord(s.done, s.stream);
-                if (e != hipSuccess) {
-                    rc = hip_fail(e, "batch submission");
-                    break;
-                }
-                s.busy = true;
+                if ((rc = launch_batch(sv, s)) != ASTC_OK) break;
                 ++inflight;
                 ++st.batches;
                 batched_reqs += (int64_t)s.reqs.size();
@@ -494,11 +565,138 @@ int astc_server_run_trace(astc_server* sv, const astc_request* reqs, int n, doub
             last_loop = now();
         }
         for (Slot& s : sv->slots) (void)hipStreamSynchronize(s.stream);
-        if (rc != ASTC_OK) return rc;
+        if (rc != ASTC_OK) {
+            // the trace owned the whole ring (no online requests): start it afresh
+            ring = Ring(sv->cfg.hbm_budget_bytes);
+            for (Slot& s : sv->slots) s.busy = false;
+            return rc;
+        }
         st.wall_s = n ? last_done - reqs[0].arrival_s : 0.0;
         st.mean_batch_requests = st.batches ? double(batched_reqs) / st.batches : 0.0;
         st.peak_ring_bytes = ring.peak();
         if (stats) *stats = st;
+        return ASTC_OK;
+    });
+}
+
+int astc_server_reserve(astc_server* sv, const astc_image_desc* images, int nimages, void** src_ring,
+                        int64_t* ticket) {
+    return guarded([&]() -> int {
+        if (!sv || !src_ring || !ticket) return fail(ASTC_ERR_INVALID_ARG, "NULL argument");
+        astc_request r{};
+        r.images = images;
+        r.nimages = nimages;
+        r.src_dev = sv->ring_base;  // placed in the ring below
+        Online rec;
+        int rc = geometry(r, &rec.geo);
+        if (rc != ASTC_OK) return rc;
+        if (ring_bytes(rec.geo) > sv->cfg.hbm_budget_bytes)
+            return fail(ASTC_ERR_BUDGET, "the request is larger than the HBM budget");
+        if (!sv->ring.alloc(ring_bytes(rec.geo), &rec.off))
+            return fail(ASTC_ERR_BUDGET, "no ring space now: complete batches (astc_server_poll) and retry");
+        const int64_t t = sv->next_ticket++;
+        *src_ring = sv->ring_base + rec.off;
+        *ticket = t;
+        sv->online.emplace(t, std::move(rec));
+        return ASTC_OK;
+    });
+}
+
+int astc_server_submit(astc_server* sv, int64_t ticket, void* stream) {
+    return guarded([&]() -> int {
+        if (!sv) return fail(ASTC_ERR_INVALID_ARG, "server is NULL");
+        auto it = sv->online.find(ticket);
+        if (it == sv->online.end() || it->second.state != Online::RESERVED)
+            return fail(ASTC_ERR_INVALID_ARG, "not a reserved ticket");
+        hipError_t e = hipSetDevice(sv->ctx->device);
+        if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+        if (stream) {
+            hipEvent_t ev = nullptr;
+            if ((e = take_event(sv, &ev)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+            if ((e = hipEventRecord(ev, static_cast<hipStream_t>(stream))) != hipSuccess) {
+                give_event(sv, ev);
+                return hip_fail(e, "hipEventRecord");
+            }
+            it->second.ready = ev;
+        }
+        it->second.state = Online::SUBMITTED;
+        sv->submitted.push_back(ticket);
+        return ASTC_OK;
+    });
+}
+
+int astc_server_poll(astc_server* sv, int64_t* done, int max_done, int* ndone) {
+    return guarded([&]() -> int {
+        if (!sv || !ndone || max_done < 0 || (max_done > 0 && !done))
+            return fail(ASTC_ERR_INVALID_ARG, "NULL argument");
+        *ndone = 0;
+        hipError_t e = hipSetDevice(sv->ctx->device);
+        if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+        for (Slot& s : sv->slots) {
+            if (!s.busy || s.tickets.empty()) continue;
+            e = hipEventQuery(s.done);
+            if (e == hipErrorNotReady) continue;
+            if (e != hipSuccess) return hip_fail(e, "batch event");
+            for (int64_t t : s.tickets) {
+                sv->online[t].state = Online::DONE;
+                sv->completed.push_back(t);
+            }
+            s.tickets.clear();
+            s.busy = false;
+        }
+        for (Slot& s : sv->slots) {
+            if (s.busy || sv->submitted.empty()) continue;
+            sv->entries.clear();
+            s.tickets.clear();
+            while (!sv->submitted.empty() && (int)s.tickets.size() < sv->cfg.max_batch_requests) {
+                const int64_t t = sv->submitted.front();
+                sv->submitted.pop_front();
+                Online& rec = sv->online[t];
+                sv->entries.push_back(BatchEntry{&rec.geo, rec.off, rec.ready, nullptr, nullptr});
+                s.tickets.push_back(t);
+            }
+            int rc = launch_batch(sv, s);
+            if (rc != ASTC_OK) {
+                // put the batch back in front of the queue, in order
+                for (auto it = s.tickets.rbegin(); it != s.tickets.rend(); ++it) sv->submitted.push_front(*it);
+                s.tickets.clear();
+                return rc;
+            }
+            for (int64_t t : s.tickets) {
+                Online& rec = sv->online[t];
+                rec.state = Online::IN_FLIGHT;
+                give_event(sv, rec.ready);  // the wait is enqueued: the event may be recorded again
+                rec.ready = nullptr;
+            }
+        }
+        while (*ndone < max_done && !sv->completed.empty()) {
+            done[(*ndone)++] = sv->completed.front();
+            sv->completed.pop_front();
+        }
+        return ASTC_OK;
+    });
+}
+
+int astc_server_blocks(astc_server* sv, int64_t ticket, const void** blocks_dev, int64_t* nbytes) {
+    if (!sv || !blocks_dev) return fail(ASTC_ERR_INVALID_ARG, "NULL argument");
+    auto it = sv->online.find(ticket);
+    if (it == sv->online.end() || it->second.state != Online::DONE)
+        return fail(ASTC_ERR_INVALID_ARG, "not a completed ticket");
+    *blocks_dev = sv->ring_base + it->second.off + it->second.geo.src_bytes;
+    if (nbytes) *nbytes = it->second.geo.out_bytes;
+    return ASTC_OK;
+}
+
+int astc_server_release(astc_server* sv, int64_t ticket) {
+    return guarded([&]() -> int {
+        if (!sv) return fail(ASTC_ERR_INVALID_ARG, "server is NULL");
+        auto it = sv->online.find(ticket);
+        if (it == sv->online.end() ||
+            (it->second.state != Online::RESERVED && it->second.state != Online::DONE))
+            return fail(ASTC_ERR_INVALID_ARG, "only a reserved (not submitted) or completed ticket can be released");
+        sv->ring.release(it->second.off);
+        give_event(sv, it->second.ready);
+        sv->online.erase(it);
         return ASTC_OK;
     });
 }

@@ -379,6 +379,44 @@ class Server:
                                           ctypes.byref(st)))
         return lat, st.as_dict()
 
+    # ---- online serving with zero-copy admission ----
+    def reserve(self, images):
+        """astc_server_reserve for a PackedImages layout: returns (ticket, ring),
+        where ring is a uint8 device tensor over the ring space for the sources
+        (write them there, then submit), or None when the ring has no room now."""
+        src = ctypes.c_void_p()
+        ticket = ctypes.c_int64()
+        rc = lib().astc_server_reserve(self._h, images.descs, images.nimages, ctypes.byref(src), ctypes.byref(ticket))
+        if rc == -6 and b"no ring space now" in lib().astc_last_error():
+            return None
+        check(rc)
+        return ticket.value, _device_view(src.value, images.buffer.numel(), images.buffer.device)
+
+    def submit(self, ticket, stream=None):
+        """Queue a reserved request; its sources are in place once the work
+        queued on `stream` (default: the current stream) so far has run."""
+        check(lib().astc_server_submit(self._h, int(ticket), _stream_handle(stream)))
+
+    def poll(self, max_done=1024):
+        """Complete finished batches and launch new ones; returns the tickets
+        completed since the last call (at most max_done)."""
+        done = (ctypes.c_int64 * max_done)()
+        n = ctypes.c_int()
+        check(lib().astc_server_poll(self._h, done, int(max_done), ctypes.byref(n)))
+        return list(done[:n.value])
+
+    def blocks(self, ticket, device=None):
+        """A completed request's blocks in the ring (uint8 device tensor view,
+        image after image), valid until release(ticket)."""
+        import torch
+        ptr = ctypes.c_void_p()
+        nbytes = ctypes.c_int64()
+        check(lib().astc_server_blocks(self._h, int(ticket), ctypes.byref(ptr), ctypes.byref(nbytes)))
+        return _device_view(ptr.value, nbytes.value, device or torch.device("cuda", torch.cuda.current_device()))
+
+    def release(self, ticket):
+        check(lib().astc_server_release(self._h, int(ticket)))
+
     def close(self):
         if self._h:
             lib().astc_server_destroy(self._h)
@@ -389,6 +427,20 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+class _DeviceBytes:
+    """A device address range described by __cuda_array_interface__ (a view,
+    owned elsewhere), for torch.as_tensor."""
+
+    def __init__(self, ptr, nbytes):
+        self.__cuda_array_interface__ = {"shape": (int(nbytes),), "typestr": "|u1", "data": (int(ptr), False),
+                                         "version": 2, "strides": None}
+
+
+def _device_view(ptr, nbytes, device):
+    import torch
+    return torch.as_tensor(_DeviceBytes(ptr, nbytes), device=device)
 
 
 _default = {}

@@ -42,6 +42,8 @@ FP32_PEAK_TFLOPS = 157.3
 # 8(2C(2C-1) + 2(2C-1) + 2 + C), min/max N(3C+1) + 8C + 11, endpoints 2C,
 # axis 6C, projections N(3C+1) (4x4) or G(10C+1) (6x6), renormalise 3 + 2G,
 # quantise 2G, with C active channels, N texels and G = 16 grid points.
+# server tuning for the saturation runs: (blocks_per_lane, workgroups_per_cu, prefetch, split_modes)
+SATURATION_TUNING = (8, 1, 0, 1)
 FLOPS_PER_BLOCK = {"rgb4x4": 1103, "rgba6x6": 2896, "normal4x4": 675, "rgba4x4": 1628, "rgb6x6": 1999}
 
 
@@ -267,8 +269,54 @@ def serving_sweep(enc, rates=(1000, 2000, 5000, 10000, 20000, 50000), seconds=0.
                         "batches": st["batches"], "mean_batch_requests": round(st["mean_batch_requests"], 2),
                         "peak_inflight": st["peak_inflight"], "max_loop_gap_ms": round(st["max_loop_gap_s"] * 1e3, 3)})
     srv.close()
+    # saturation with the batched form's throughput tuning (8 blocks per lane,
+    # one 6x6 workgroup per CU, split kernels), which the low-rate sweep's
+    # latency-oriented default trades away
+    srv = astc.Server(enc, 64 << 30, max_inflight=3, max_batch_requests=256, tuning=SATURATION_TUNING)
+    srv.run_trace([(pool[i % pool_size], 0.0, None) for i in range(4 * pool_size)])
+    sat = saturation(srv, pool, rng)
+    sat["tuning"] = dict(zip(("blocks_per_lane", "workgroups_per_cu", "prefetch", "split_modes"), SATURATION_TUNING))
+    srv.close()
     return {"workload": f"Poisson arrivals, {pool_size} mip-chained textures of the mixed distribution (seed 1)",
-            "hbm_budget_gb": 64, "max_inflight_batches": 3, "sweep": results}
+            "hbm_budget_gb": 64, "max_inflight_batches": 3, "sweep": results, "saturation": sat}
+
+
+def saturation(srv, pool, rng, n=1500):
+    """Sustained throughput with n requests all waiting at once, two ways:
+    copy-in (astc_server_run_trace: each request's sources are copied into the
+    ring on admission) and zero-copy (online calls: the sources were written
+    into reserved ring space beforehand, as a producer writing its texture there
+    would, and every request submitted; the timed part is the server draining
+    them: batching and encoding, from the first poll to the last completion)."""
+    import torch
+    picks = rng.integers(0, len(pool), n)
+    texels = int(sum(pool[k].texels for k in picks))
+    lat, st = srv.run_trace([(pool[k], 0.0, None) for k in picks])
+    copy_in = {"requests": n, "mtexels_s": round(texels / st["wall_s"] / 1e6, 1), "wall_ms": round(st["wall_s"] * 1e3, 2),
+               "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3), "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+               "mean_batch_requests": round(st["mean_batch_requests"], 1)}
+    tickets = []
+    for k in picks:
+        ticket, ring = srv.reserve(pool[k])
+        ring.copy_(pool[k].buffer)
+        tickets.append(ticket)
+    torch.cuda.synchronize()
+    for t in tickets:
+        srv.submit(t, None)
+    t0 = time.perf_counter()
+    done, finish = 0, {}
+    while done < n:
+        for t in srv.poll():
+            finish[t] = time.perf_counter() - t0
+            done += 1
+    wall = max(finish.values())
+    lat = np.array([finish[t] for t in tickets])
+    for t in tickets:
+        srv.release(t)
+    zero_copy = {"requests": n, "mtexels_s": round(texels / wall / 1e6, 1), "wall_ms": round(wall * 1e3, 2),
+                 "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                 "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3)}
+    return {"copy_in": copy_in, "zero_copy": zero_copy, "texels": texels}
 
 
 def _time_runs(fn, min_s, min_runs=5, max_runs=2000):

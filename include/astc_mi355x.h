@@ -264,6 +264,34 @@ int astc_server_run_trace(astc_server* server, const astc_request* requests, int
                           astc_server_stats* stats);
 int astc_server_destroy(astc_server* server);
 
+/* ---- online serving with zero-copy admission --------------------------------
+ * The caller writes each request's sources straight into ring space the server
+ * hands out, so admission copies nothing, and reads the blocks where they were
+ * encoded.  The server is driven by astc_server_poll from the caller's own
+ * loop (one thread at a time; astc_server_run_trace refuses to run while online
+ * requests are outstanding).  No reference counterpart. */
+
+/* Reserve ring space for a request (sources laid out as for astc_request.src_dev,
+ * then its blocks).  *src_ring: where the caller writes the sources (device
+ * memory); *ticket: the request's id.  ASTC_ERR_BUDGET when the ring has no room
+ * now (complete batches with astc_server_poll and release tickets, then retry)
+ * or the request exceeds the whole budget. */
+int astc_server_reserve(astc_server* server, const astc_image_desc* images, int nimages, void** src_ring,
+                        int64_t* ticket);
+/* Queue a reserved request for the next batch.  Its sources are taken to be in
+ * place once the work queued on `stream` so far has run (NULL: already in place). */
+int astc_server_submit(astc_server* server, int64_t ticket, void* stream);
+/* Complete finished batches and launch new ones from the submitted requests
+ * (up to max_inflight batches of up to max_batch_requests each), without
+ * waiting.  Writes up to max_done completed tickets to done[] and their count
+ * to *ndone; completions beyond max_done are reported by later calls. */
+int astc_server_poll(astc_server* server, int64_t* done, int max_done, int* ndone);
+/* A completed request's blocks in the ring, image after image with no gaps
+ * (16-byte aligned), valid until the ticket is released. */
+int astc_server_blocks(astc_server* server, int64_t ticket, const void** blocks_dev, int64_t* nbytes);
+/* Return a completed (or reserved, never submitted) request's ring space. */
+int astc_server_release(astc_server* server, int64_t ticket);
+
 /* ---- image loading for the command-line drop-in ----------------------------
  * Replaces load_tex's stbi_load(path, ..., STBI_rgb_alpha) with
  * stbi_set_flip_vertically_on_load(1) (main.cpp:19-56) for PNG input: every PNG

@@ -215,3 +215,132 @@ def test_exact_size_buffers_and_tight_block_layout(enc, pool):
         np.testing.assert_array_equal(dst.cpu().numpy(), torch.cat([w.view(-1) for w in want]).cpu().numpy())
         assert bool((full[pk.out_bytes:] == 0xA5).all())
     srv.close()
+
+
+def _online_run(srv, work, timeout_s=60.0):
+    """Submit (PackedImages, want) pairs through the online calls: the sources are
+    written into the ring space by a device copy on the current stream, then
+    submitted; returns the tickets in submission order after every request has
+    completed and been checked, and releases them."""
+    import time
+    tickets, pending = [], {}
+    queue = list(work)
+    t_end = time.time() + timeout_s
+    while queue or pending:
+        assert time.time() < t_end, "online requests did not complete"
+        while queue:
+            pk, want = queue[0]
+            got = srv.reserve(pk)
+            if got is None:
+                break
+            ticket, ring = got
+            ring.copy_(pk.buffer)
+            srv.submit(ticket)
+            pending[ticket] = (pk, want)
+            tickets.append(ticket)
+            queue.pop(0)
+        for t in srv.poll():
+            pk, want = pending.pop(t)
+            blocks = srv.blocks(t)
+            check_request(pk, want, blocks)
+            srv.release(t)
+    return tickets
+
+
+def test_online_zero_copy_alone_and_in_the_mix(enc, pool):
+    srv = astc.Server(enc, 256 << 20, max_inflight=3, max_batch_requests=16)
+    # alone
+    pk, want = pool[2]
+    ticket, ring = srv.reserve(pk)
+    ring.copy_(pk.buffer)
+    srv.submit(ticket)
+    done = []
+    while not done:
+        done = srv.poll()
+    assert done == [ticket]
+    check_request(pk, want, srv.blocks(ticket))
+    alone = srv.blocks(ticket).clone()
+    srv.release(ticket)
+    # in a mix of 300 requests, several batches in flight
+    rng = np.random.default_rng(3)
+    picks = rng.integers(0, len(pool), 300)
+    tickets = _online_run(srv, [pool[k] for k in picks])
+    assert len(tickets) == 300 and len(set(tickets)) == 300
+    # the same request again, now among others: the same bytes
+    ticket, ring = srv.reserve(pk)
+    ring.copy_(pk.buffer)
+    srv.submit(ticket)
+    others = []
+    for k in range(5):
+        q, _ = pool[k]
+        t2, r2 = srv.reserve(q)
+        r2.copy_(q.buffer)
+        srv.submit(t2)
+        others.append(t2)
+    done = set()
+    while len(done) < 6:
+        done |= set(srv.poll())
+    assert torch.equal(srv.blocks(ticket), alone)
+    for t in [ticket] + others:
+        srv.release(t)
+    srv.close()
+
+
+def test_online_sources_written_on_another_stream(enc, pool):
+    """The server waits for the producer's stream before reading the ring: the
+    sources are written by a copy on a side stream right before submission."""
+    srv = astc.Server(enc, 256 << 20, max_inflight=2, max_batch_requests=8)
+    side = torch.cuda.Stream()
+    work = []
+    for k in range(24):
+        pk, want = pool[k % len(pool)]
+        ticket, ring = srv.reserve(pk)
+        ring.zero_()
+        torch.cuda.synchronize()
+        with torch.cuda.stream(side):
+            torch.cuda._sleep(200000)  # the copy lands well after the submission
+            ring.copy_(pk.buffer)
+        srv.submit(ticket, side)
+        work.append((ticket, pk, want))
+    done = set()
+    while len(done) < len(work):
+        done |= set(srv.poll())
+    torch.cuda.synchronize()
+    for ticket, pk, want in work:
+        check_request(pk, want, srv.blocks(ticket))
+        srv.release(ticket)
+    srv.close()
+
+
+def test_online_ring_full_and_ticket_rules(enc, pool):
+    pk, want = max(pool, key=lambda p: p[0].buffer.numel() + p[0].out_bytes)
+    rb = astc.lib().astc_request_bytes(pk.descs, pk.nimages)
+    srv = astc.Server(enc, rb + 1024, max_inflight=1, max_batch_requests=4)
+    ticket, ring = srv.reserve(pk)
+    assert srv.reserve(pk) is None  # no room until the first is released
+    with pytest.raises(astc.AstcError, match="not a completed ticket"):
+        srv.blocks(ticket)
+    ring.copy_(pk.buffer)
+    srv.submit(ticket)
+    with pytest.raises(astc.AstcError, match="not a reserved ticket"):
+        srv.submit(ticket)
+    # run_trace refuses while online requests are outstanding
+    with pytest.raises(astc.AstcError, match="online requests"):
+        srv.run_trace([(pk, 0.0, None)])
+    done = []
+    while not done:
+        done = srv.poll()
+    check_request(pk, want, srv.blocks(ticket))
+    srv.release(ticket)
+    with pytest.raises(astc.AstcError):
+        srv.release(ticket)
+    # a reserved ticket may be given back unsubmitted
+    t2, _ = srv.reserve(pk)
+    srv.release(t2)
+    with pytest.raises(astc.AstcError) as e:
+        astc.Server(enc, 1024, max_inflight=1, max_batch_requests=1).reserve(pk)
+    assert e.value.code == -6
+    # and the trace form works again on the emptied ring
+    lat, st = srv.run_trace([(pk, 0.0, None)])
+    assert st["requests"] == 1
+    srv.close()
