@@ -486,14 +486,17 @@ __device__ __forceinline__ void covariance_chain(const float x[NX][4], const flo
 // e1 - e0; negation is exact and round-to-nearest is symmetric in sign, so
 // every later quantity of the swapped block is the unswapped one with its sign
 // flipped.  The swap is carried as that flag instead of moving values.
+// With NC < 4 the inactive channels' endpoints are 255, except alpha, which is
+// `alpha` (the block's constant alpha for the dynamic three-channel path).
 template <bool ALPHA, int NC>
 __device__ __forceinline__ bool endpoints(const float v[4], float lo, float hi, const float mean[4], float e0[4],
-                                          float e1[4], uint32_t e0q[4], uint32_t e1q[4]) {
+                                          float e1[4], uint32_t e0q[4], uint32_t e1q[4], float alpha = 255.0f) {
     const float mv = dotn<NC>(mean, v);
     lo = lo - mv;
     hi = hi - mv;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) e0[c] = e1[c] = 255.0f;
+    for (int c = 0; c < 3; ++c) e0[c] = e1[c] = 255.0f;
+    e0[3] = e1[3] = alpha;
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
         e0[c] = clamp255(fma_(v[c], lo, mean[c]));
@@ -568,10 +571,36 @@ __device__ __forceinline__ void quantize_weights(const float proj[NG], float min
     }
 }
 
+// A four-channel block whose alpha is the same byte c in every texel, with
+// UNORM value c itself (130 of the 256 bytes, 0 and 255 among them), encodes to
+// the same bytes through the three-channel path: the alpha mean is exactly c
+// (sums of 18 and 36 copies of an integer and the scale by fl(1/36) or
+// fl(1/16) are exact), so every alpha difference from it is +-0, the alpha
+// covariance row and the axis' alpha component are +-0, the alpha endpoints
+// are c, and every term the four-channel computation adds for alpha is a
+// signed zero, which changes no rounded result.  words: the block's packed
+// RGBA8 texels; unorm: the UNORM conversion table.
+template <int N>
+__device__ __forceinline__ bool flat_alpha(const uint32_t words[N], const float* unorm) {
+    // alpha is the top byte, so the largest and smallest words carry the
+    // largest and smallest alpha
+    uint32_t hi = words[0], lo = words[0];
+#pragma unroll
+    for (int k = 1; k < N; ++k) {
+        hi = max(hi, words[k]);
+        lo = min(lo, words[k]);
+    }
+    const uint32_t a = words[0] >> 24;
+    return ((hi ^ lo) >> 24) == 0 && unorm[a] == float(a);
+}
+
 // RESCALE: the power iteration's rescale interval, 3 or 1 (every product); both
-// give the same blocks (see principal_axis).
+// give the same blocks (see principal_axis).  alpha: the constant alpha of a
+// four-channel block encoded by the three-channel path (NC = 3; see
+// flat_alpha), else 255.
 template <int DIM, bool ALPHA, int NC = 4, int METHOD = METHOD_PCA, int RESCALE = 3>
-__device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], const PackTables* pack) {
+__device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], const PackTables* pack,
+                                                 float alpha = 255.0f) {
     static_assert(NC >= 2 && NC <= 4, "two to four active channels");
     constexpr int N = DIM * DIM;
     constexpr int H = SumHalf<DIM>::value;
@@ -664,7 +693,7 @@ __device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], co
     }
     float e0[4], e1[4];
     uint32_t e0q[4], e1q[4];
-    const bool swap = endpoints<ALPHA, NC>(v, lo, hi, mean, e0, e1, e0q, e1q);
+    const bool swap = endpoints<ALPHA, NC>(v, lo, hi, mean, e0, e1, e0q, e1q, alpha);
 
     // calculate_normal_weights (:316-372) + quantize_weights (:374-382)
     constexpr int KC = (NC == 4 && !ALPHA) ? 3 : NC;  // RGB mode: endpoint alphas are equal

@@ -86,31 +86,45 @@ struct BatchRescale { static constexpr int value = (SRGB < 0 && DIM == 6) ? 1 : 
 // loads).  With prefetch the next step's texels are loaded as soon as the
 // current ones are converted (into the same registers), so the loads are in
 // flight while the block is encoded; without, after the block is stored.
-template <int DIM, bool ALPHA, int SRGB, int CH, int METHOD = METHOD_PCA>
+// FLAT: a wave whose blocks all have a flat alpha (flat_alpha) is encoded by
+// the three-channel path (same bytes); the test costs a few percent where few
+// waves are flat, so it is a plain-kernel tuning, compiled separately.
+template <int DIM, bool ALPHA, int SRGB, int CH, int METHOD = METHOD_PCA, bool FLAT = false>
 __device__ __forceinline__ void encode_loaded(const FetchArgs& f, uint32_t* dst, int first, int count, bool prefetch,
                                               const float* lut, const PackTables* trit,
                                               uint32_t cur[Layout<DIM, CH>::WORDS]) {
     const int stride = blockDim.x;
+    constexpr int RESCALE = BatchRescale<DIM, SRGB>::value;
 #pragma unroll 1
     for (int o = threadIdx.x;; o += stride) {
         const bool has_next = o + stride < count;
+        constexpr bool kFlat = FLAT && CH == CH_ALL && METHOD == METHOD_PCA;
+        const bool flat = kFlat && __all(flat_alpha<DIM * DIM>(cur, lut + 256));
         float x[DIM * DIM][4];
         unpack_texels<DIM, SRGB, CH, METHOD == METHOD_MAX_ACCUMULATION>(cur, f, lut, x);
         // the packed words are dead once converted: the next block's loads reuse them
         if (has_next && prefetch) load_block_id<DIM, CH>(f, first + o + stride, cur);
-        store_block(dst, first + o, encode_block<DIM, ALPHA, CH, METHOD, BatchRescale<DIM, SRGB>::value>(x, trit));
+        Block128 b;
+        if (kFlat && flat) {
+            // keeps the two encoders' code apart: merged, their live values exceed the register file
+            __builtin_amdgcn_sched_barrier(0);
+            b = encode_block<DIM, ALPHA, CH_RGB8, METHOD, RESCALE>(x, trit, x[0][3]);
+        } else {
+            b = encode_block<DIM, ALPHA, CH, METHOD, RESCALE>(x, trit);
+        }
+        store_block(dst, first + o, b);
         if (!has_next) break;
         if (!prefetch) load_block_id<DIM, CH>(f, first + o + stride, cur);
     }
 }
 
-template <int DIM, bool ALPHA, int SRGB, int CH, int METHOD = METHOD_PCA>
+template <int DIM, bool ALPHA, int SRGB, int CH, int METHOD = METHOD_PCA, bool FLAT = false>
 __device__ __forceinline__ void encode_chunk(const FetchArgs& f, uint32_t* dst, int first, int count, bool prefetch,
                                              const float* lut, const PackTables* trit) {
     if (int(threadIdx.x) >= count) return;
     uint32_t cur[Layout<DIM, CH>::WORDS];
     load_block_id<DIM, CH>(f, first + threadIdx.x, cur);
-    encode_loaded<DIM, ALPHA, SRGB, CH, METHOD>(f, dst, first, count, prefetch, lut, trit, cur);
+    encode_loaded<DIM, ALPHA, SRGB, CH, METHOD, FLAT>(f, dst, first, count, prefetch, lut, trit, cur);
 }
 
 // Minimum waves per SIMD the register budget must allow: 4x4 encoders fit in
@@ -119,7 +133,7 @@ __device__ __forceinline__ void encode_chunk(const FetchArgs& f, uint32_t* dst, 
 template <int DIM>
 struct PlainWaves { static constexpr int value = DIM == 4 ? 4 : 2; };
 
-template <int DIM, bool ALPHA, bool SRGB, int CH, int METHOD>
+template <int DIM, bool ALPHA, bool SRGB, int CH, int METHOD, bool FLAT = false>
 __global__ void __launch_bounds__(kPlainThreads)
 __attribute__((amdgpu_waves_per_eu(PlainWaves<DIM>::value))) encode_plain_kernel(PlainArgs a, int bpl, int prefetch) {
     __shared__ float lut[512];
@@ -129,7 +143,7 @@ __attribute__((amdgpu_waves_per_eu(PlainWaves<DIM>::value))) encode_plain_kernel
     const int chunk = kPlainThreads * bpl;
     const int first = blockIdx.x * chunk;
     const int count = min(chunk, a.nblocks - first);
-    encode_chunk<DIM, ALPHA, SRGB ? 1 : 0, CH, METHOD>(a.f, a.dst, first, count, prefetch != 0, lut, &trit);
+    encode_chunk<DIM, ALPHA, SRGB ? 1 : 0, CH, METHOD, FLAT>(a.f, a.dst, first, count, prefetch != 0, lut, &trit);
 }
 
 // ---------------------------------------------------------------------------
@@ -368,6 +382,16 @@ static void launch_plain_ch(const PlainArgs& a, int64_t nblocks, const PlainTuni
     }
     const int chunk = kPlainThreads * t.blocks_per_lane;
     const dim3 grid(unsigned((nblocks + chunk - 1) / chunk));
+    constexpr bool kFlatCapable = CH == CH_ALL && METHOD == METHOD_PCA;
+    if (kFlatCapable && t.flat_alpha) {
+        if (srgb)
+            hipLaunchKernelGGL((encode_plain_kernel<DIM, ALPHA, true, CH, METHOD, kFlatCapable>), grid,
+                               dim3(kPlainThreads), 0, stream, a, t.blocks_per_lane, t.prefetch);
+        else
+            hipLaunchKernelGGL((encode_plain_kernel<DIM, ALPHA, false, CH, METHOD, kFlatCapable>), grid,
+                               dim3(kPlainThreads), 0, stream, a, t.blocks_per_lane, t.prefetch);
+        return;
+    }
     if (srgb)
         hipLaunchKernelGGL((encode_plain_kernel<DIM, ALPHA, true, CH, METHOD>), grid, dim3(kPlainThreads), 0, stream, a,
                            t.blocks_per_lane, t.prefetch);
@@ -407,6 +431,7 @@ PlainTuning default_plain_tuning(const astc_image_desc& d, int64_t nblocks, int 
     PlainTuning t;
     t.lanes_per_block = 1;
     t.prefetch = 0;
+    t.flat_alpha = 0;
     t.blocks_per_lane = plain_blocks_per_lane(nblocks, d.block_dim, t.lanes_per_block, num_cus);
     return t;
 }

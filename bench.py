@@ -89,7 +89,7 @@ def time_plain(enc, wl, steps, warmup, tuning=None):
         fn, args = astc.lib().astc_encode, (ctx, dref, sp, dp, st)
     else:
         from astc_encoder_amd._capi import PlainTuning
-        t = PlainTuning(*tuning, 0)
+        t = PlainTuning(*(tuple(tuning) + (0,) * (4 - len(tuning))))
         fn, args = astc.lib().astc_encode_tuned, (ctx, dref, sp, dp, ctypes.byref(t), st)
 
     def launch():
@@ -126,6 +126,32 @@ def time_plain(enc, wl, steps, warmup, tuning=None):
     return dict(img=img, blocks=out.cpu().numpy(), wall_s=wall, kernel_ms=kernel_ms, step_event_ms=step_event_ms,
                 texels=texels,
                 nblocks=nblocks, mtexels_s=texels * steps / wall / 1e6, psnr_db=gpu_psnr(enc, wl, src, out))
+
+
+def opaque_autotuned(enc, steps, warmup):
+    """The headline's colours with an opaque alpha (every byte 255, what the
+    command line's RGBA8 load of an opaque image gives), RGBA mode: plain
+    launches at the default tuning and at the plain autotuner's choice."""
+    import torch
+    import astc_encoder_amd as astc
+    from astc_encoder_amd import synth
+
+    def opaque(seed, w, h):
+        img = synth.noise_rgba8(seed, w, h)
+        img[..., 3] = 255
+        return img
+
+    wl = dict(name="opaque RGBA8 sRGB 4096x4096 ASTC 6x6", gen=opaque, w=4096, h=4096, dim=6,
+              flags=astc.FLAG_ALPHA | astc.FLAG_SRGB)
+    best = enc.autotune_plain(torch.from_numpy(opaque(0, 4096, 4096)).cuda(), 6, wl["flags"], reps=20)
+    default = time_plain(enc, wl, steps, warmup)
+    tuned = time_plain(enc, wl, steps, warmup,
+                       (best["lanes_per_block"], best["blocks_per_lane"], best["prefetch"], best["flat_alpha"]))
+    same = bool(np.array_equal(default["blocks"], tuned["blocks"]))
+    return {"workload": wl["name"], "default_mtexels_s": round(default["mtexels_s"], 1),
+            "default_kernel_ms": round(default["kernel_ms"], 4), "autotuned": best,
+            "autotuned_mtexels_s": round(tuned["mtexels_s"], 1), "autotuned_kernel_ms": round(tuned["kernel_ms"], 4),
+            "same_blocks": same}
 
 
 def time_host_resident(enc, wl, steps):
@@ -400,7 +426,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the mixed-texture batched vs plain run")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving sweep")
     ap.add_argument("--plain-tuning", default=None,
-                    help="lanes_per_block,blocks_per_lane,prefetch for the headline's plain launches "
+                    help="lanes_per_block,blocks_per_lane,prefetch[,flat_alpha] for the headline's plain launches "
                          "(default: the library's choice)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run 3 device copies of 256 MiB (a known-size kernel for scaling HBM counters)")
@@ -435,6 +461,7 @@ def main():
                            "psnr_db": e["psnr_db"],
                            "valu_tflops": round(FLOPS_PER_BLOCK[wl["flops_key"]] * e["nblocks"] /
                                                 (e["kernel_ms"] * 1e-3) / 1e12, 2)}
+        extras["opaque_rgba8_6x6_autotuned"] = opaque_autotuned(enc, max(10, args.steps), args.warmup)
         extras["host_resident_headline"] = time_host_resident(enc, head, 20)
         if not args.no_batched:
             extras["mixed_textures_batched_vs_plain"] = batched_vs_plain(enc, max(5, args.steps // 5), 2)

@@ -100,12 +100,16 @@ int astc_encode(astc_ctx* ctx, const astc_image_desc* desc, const void* src_dev,
  * other block sizes and the max-accumulation search run 1).  blocks_per_lane:
  * 1, 2, 4 or 8 (a workgroup covers 256 / lanes_per_block x this many blocks),
  * or 0 for the library's choice.  prefetch: 1 issues the next block's loads
- * before the current block is encoded.  Every tuning gives the same blocks. */
+ * before the current block is encoded.  flat_alpha: 1 tests each wave of
+ * four-channel blocks (RGBA8 sources, one lane per block) for an alpha that is
+ * the same integer-valued byte in every texel of every block, and encodes such
+ * waves with the three-channel encoder (faster on opaque textures, a little
+ * slower where few waves qualify).  Every tuning gives the same blocks. */
 typedef struct astc_plain_tuning {
     int32_t lanes_per_block;
     int32_t blocks_per_lane;
     int32_t prefetch;
-    int32_t reserved;
+    int32_t flat_alpha;
 } astc_plain_tuning;
 
 /* astc_encode with an explicit plain-kernel tuning (astc_encode uses the

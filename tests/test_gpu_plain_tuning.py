@@ -109,8 +109,9 @@ def test_autotune_picks_a_valid_tuning(enc):
     tex = torch.from_numpy(img).cuda()
     best = enc.autotune_plain(tex, 6, flags, reps=3)
     assert best["lanes_per_block"] in (1, 2) and best["blocks_per_lane"] in (1, 2, 4, 8)
-    assert best["prefetch"] in (0, 1)
-    got = tuned(enc, tex, 6, flags, (best["lanes_per_block"], best["blocks_per_lane"], best["prefetch"]))
+    assert best["prefetch"] in (0, 1) and best["flat_alpha"] in (0, 1)
+    got = tuned(enc, tex, 6, flags, (best["lanes_per_block"], best["blocks_per_lane"], best["prefetch"],
+                                     best["flat_alpha"]))
     assert_same(got, orc.oracle_encode(img, 6, flags), "autotuned")
     # 4x4 images and the max-accumulation search have one lane per block only
     best4 = enc.autotune_plain(tex, 4, flags, reps=1)
@@ -119,7 +120,7 @@ def test_autotune_picks_a_valid_tuning(enc):
     assert bestm["lanes_per_block"] == 1
 
 
-@pytest.mark.parametrize("tuning", [(3, 0, 0), (0, 0, 0), (2, 3, 0), (2, 16, 0), (1, 0, 2), (2, -1, 0)])
+@pytest.mark.parametrize("tuning", [(3, 0, 0), (0, 0, 0), (2, 3, 0), (2, 16, 0), (1, 0, 2), (2, -1, 0), (1, 0, 0, 2)])
 def test_out_of_range_tunings_are_rejected(enc, tuning):
     tex = torch.zeros((12, 12, 4), dtype=torch.uint8, device="cuda")
     with pytest.raises(astc.AstcError, match="plain tuning out of range"):
