@@ -38,11 +38,12 @@ class ImageDesc(ctypes.Structure):
 class Tuning(ctypes.Structure):
     """astc_tuning"""
     _fields_ = [("blocks_per_lane", ctypes.c_int32), ("workgroups_per_cu", ctypes.c_int32),
-                ("prefetch", ctypes.c_int32), ("split_modes", ctypes.c_int32)]
+                ("prefetch", ctypes.c_int32), ("split_modes", ctypes.c_int32), ("lanes_per_block", ctypes.c_int32)]
 
     def as_dict(self):
         return {"blocks_per_lane": self.blocks_per_lane, "workgroups_per_cu": self.workgroups_per_cu,
-                "prefetch": self.prefetch, "split_modes": self.split_modes}
+                "prefetch": self.prefetch, "split_modes": self.split_modes,
+                "lanes_per_block": max(1, self.lanes_per_block)}
 
 
 class PlainTuning(ctypes.Structure):

@@ -25,7 +25,7 @@ struct astc_batch {
     };
     std::vector<PlainImage> plain;
     int64_t blocks = 0, texels = 0;
-    astc_tuning tuning{4, 2, 1, 1};
+    astc_tuning tuning{4, 2, 1, 1, 1};
     // device work list
     BatchLevel* d_levels = nullptr;
     BatchSeg* d_segs = nullptr;
@@ -191,6 +191,7 @@ int build(astc_batch* b) {
     a.tables = b->ctx->d_tables;
     a.prefetch = b->tuning.prefetch;
     BatchLaunch l;
+    l.lanes = batch_lanes(b->tuning);
     l.grid = std::max(1, b->num_cus * b->tuning.workgroups_per_cu);
     // the 4x4 kernel's own occupancy (five waves per SIMD), whatever the 6x6
     // kernel's: with one 6x6 workgroup per CU, three 4x4 waves share each SIMD
@@ -215,12 +216,7 @@ int build(astc_batch* b) {
     return ASTC_OK;
 }
 
-bool valid_tuning(const astc_tuning* t) {
-    const int bpl = t->blocks_per_lane;
-    return (bpl == 1 || bpl == 2 || bpl == 4 || bpl == 8) && t->workgroups_per_cu >= 1 &&
-           t->workgroups_per_cu <= 4 && (t->prefetch == 0 || t->prefetch == 1) &&
-           (t->split_modes == 0 || t->split_modes == 1);
-}
+bool valid_tuning(const astc_tuning* t) { return valid_batch_tuning(*t); }
 
 }  // namespace
 
@@ -336,12 +332,14 @@ int astc_batch_autotune(astc_batch* b, void* stream, int reps, astc_tuning* best
         astc_tuning best = b->tuning;
         float best_ms = 1e30f;
         std::vector<astc_tuning> candidates;
-        for (int split : {0, 1})
-            for (int bpl : {1, 2, 4, 8})
-                for (int wpc : {1, 2})
-                    for (int pf : {0, 1})
-                        if (!(bpl == 1 && pf == 1))  // nothing to prefetch within an item
-                            candidates.push_back(astc_tuning{bpl, wpc, pf, split});
+        for (int lanes : {1, 2})
+            for (int split : {0, 1})
+                for (int bpl : {1, 2, 4, 8})
+                    for (int wpc : {1, 2})
+                        for (int pf : {0, 1}) {
+                            if (bpl == 1 && pf == 1) continue;  // nothing to prefetch within an item
+                            candidates.push_back(astc_tuning{bpl, wpc, pf, split, lanes});
+                        }
         int rc = ASTC_OK;
         for (const astc_tuning& t : candidates) {
             if ((rc = astc_batch_set_tuning(b, &t)) != ASTC_OK) break;

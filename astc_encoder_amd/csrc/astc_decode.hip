@@ -324,6 +324,7 @@ hipError_t launch_decode(const TritQuintDecodeTable* d_tables, const void* block
     a.xblocks = (width + bw - 1) / bw;
     a.nblocks = a.xblocks * ((height + bh - 1) / bh);
     a.tables = d_tables;
+    (void)hipGetLastError();  // report this launch's errors only, not an earlier call's
     hipLaunchKernelGGL(decode_kernel, dim3(unsigned((a.nblocks + kDecThreads - 1) / kDecThreads)), dim3(kDecThreads),
                        0, stream, a);
     return hipGetLastError();

@@ -116,6 +116,7 @@ struct BatchArgs {
     int prefetch;
 };
 struct BatchLaunch {
+    int lanes = 1; // lanes per 6x6 block (1 or 2)
     int grid;      // workgroups of the (6x6 or only) persistent kernel
     int grid4;     // workgroups of the 4x4 kernel when split
     int split;     // 0: one kernel for every item; 1: 6x6 and 4x4 kernels
@@ -124,6 +125,14 @@ struct BatchLaunch {
 };
 // Launches the counter reset and the persistent kernel(s) on `stream`.
 hipError_t launch_encode_batch(const BatchArgs& a, const BatchLaunch& l, hipStream_t stream);
+// A batch tuning within range (lanes_per_block 0 counts as 1).
+inline bool valid_batch_tuning(const astc_tuning& t) {
+    const int bpl = t.blocks_per_lane;
+    return (bpl == 1 || bpl == 2 || bpl == 4 || bpl == 8) && t.workgroups_per_cu >= 1 && t.workgroups_per_cu <= 4 &&
+           (t.prefetch == 0 || t.prefetch == 1) && (t.split_modes == 0 || t.split_modes == 1) &&
+           (t.lanes_per_block >= 0 && t.lanes_per_block <= 2);
+}
+inline int batch_lanes(const astc_tuning& t) { return t.lanes_per_block == 2 ? 2 : 1; }
 inline int mode_of(int block_dim, int src_format, uint32_t flags, int width, int height) {
     const bool tiled = width % block_dim == 0 && height % block_dim == 0;
     const int ch = (flags & FLAG_NORMAL) ? MODE_CH_NORMAL

@@ -211,9 +211,38 @@ __global__ void reset_counter_kernel(int* counters) {
     if (threadIdx.x < 2 && blockIdx.x == 0) counters[threadIdx.x] = 0;
 }
 
-template <int DIM, bool ALPHA, int CH>
+// LPB = 2: 6x6 items with two lanes per block (encode_block_pair): a chunk
+// runs encode_chunk_pair; a pack of small images (at most 256 blocks) takes
+// two rounds of 128 lane pairs.
+template <int DIM, bool ALPHA, int CH, int LPB = 1>
 __device__ __forceinline__ void process_item(const BatchArgs& a, const BatchItem& item, const float* lut,
                                              const PackTables* trit) {
+    if (DIM == 6 && LPB == 2) {
+        if (item.nseg == 1) {
+            const BatchSeg sg = a.segs[item.seg_begin];
+            const BatchLevel& L = a.levels[sg.level];
+            encode_chunk_pair<ALPHA, -1, CH, 3>(L.f, L.dst, sg.first_block, sg.count, a.prefetch != 0, lut, trit);
+            return;
+        }
+        const int half = threadIdx.x & 1, pairs = blockDim.x >> 1;
+        for (int r = 0; r < 2; ++r) {
+            int o = r * pairs + (threadIdx.x >> 1);
+            for (int s = 0; s < item.nseg; ++s) {
+                const BatchSeg sg = a.segs[item.seg_begin + s];
+                if (o < sg.count) {
+                    const BatchLevel& L = a.levels[sg.level];
+                    uint32_t p[3 * Layout<6, CH>::WPR];
+                    load_half_block<CH>(L.f, sg.first_block + o, half, p);
+                    float x[18][4];
+                    unpack_texels<6, -1, CH, false, 3>(p, L.f, lut, x);
+                    store_half_block(L.dst, sg.first_block + o, half, encode_block_pair<ALPHA, CH, 3>(x, half, trit));
+                    break;
+                }
+                o -= sg.count;
+            }
+        }
+        return;
+    }
     if (item.nseg == 1) {
         const BatchSeg sg = a.segs[item.seg_begin];
         const BatchLevel& L = a.levels[sg.level];
@@ -236,13 +265,13 @@ __device__ __forceinline__ void process_item(const BatchArgs& a, const BatchItem
     }
 }
 
-template <int DIM, bool ALPHA>
+template <int DIM, bool ALPHA, int LPB>
 __device__ __forceinline__ void process_item_ch(const BatchArgs& a, const BatchItem& item, const float* lut,
                                                 const PackTables* trit) {
     switch (item.mode & MODE_CH_MASK) {
-    case MODE_CH_NORMAL: process_item<DIM, ALPHA, CH_NORMAL>(a, item, lut, trit); break;
-    case MODE_CH_RGB8: process_item<DIM, ALPHA, CH_RGB8>(a, item, lut, trit); break;
-    default: process_item<DIM, ALPHA, CH_ALL>(a, item, lut, trit); break;
+    case MODE_CH_NORMAL: process_item<DIM, ALPHA, CH_NORMAL, LPB>(a, item, lut, trit); break;
+    case MODE_CH_RGB8: process_item<DIM, ALPHA, CH_RGB8, LPB>(a, item, lut, trit); break;
+    default: process_item<DIM, ALPHA, CH_ALL, LPB>(a, item, lut, trit); break;
     }
 }
 
@@ -251,9 +280,10 @@ __device__ __forceinline__ void process_item_ch(const BatchArgs& a, const BatchI
 // item of a reduced-channel class (RGB8 tiles, normal maps): the 6x6 kernel
 // takes the front (6x6 items, then 4x4 four-channel items, which fit its
 // budget), the 4x4 kernel the rest, with the smaller budget of those two
-// classes (five waves per SIMD).
-template <int SEL>
-struct BatchWaves { static constexpr int value = SEL == 4 ? 5 : 2; };
+// classes (five waves per SIMD).  LPB = 2 (6x6 items with two lanes per
+// block) lowers the budget of the kernels that hold 6x6 items to three waves.
+template <int SEL, int LPB>
+struct BatchWaves { static constexpr int value = SEL == 4 ? 5 : LPB == 2 ? 3 : 2; };
 
 template <int SEL, bool ALPHA>
 __device__ __forceinline__ void process_item_4x4(const BatchArgs& a, const BatchItem& item, const float* lut,
@@ -271,8 +301,8 @@ __device__ __forceinline__ void process_item_4x4(const BatchArgs& a, const Batch
     }
 }
 
-template <int SEL>
-__global__ void __launch_bounds__(kPlainThreads) __attribute__((amdgpu_waves_per_eu(BatchWaves<SEL>::value)))
+template <int SEL, int LPB = 1>
+__global__ void __launch_bounds__(kPlainThreads) __attribute__((amdgpu_waves_per_eu(BatchWaves<SEL, LPB>::value)))
 encode_batch_kernel(BatchArgs a) {
     __shared__ float lut[512];
     __shared__ PackTables trit;
@@ -305,19 +335,23 @@ encode_batch_kernel(BatchArgs a) {
             process_item_4x4<SEL, true>(a, item, lut, &trit);
             break;
         case MODE_6x6_RGB:
-            if (SEL != 4) process_item_ch<6, false>(a, item, lut, &trit);
+            if (SEL != 4) process_item_ch<6, false, LPB>(a, item, lut, &trit);
             break;
         default:
-            if (SEL != 4) process_item_ch<6, true>(a, item, lut, &trit);
+            if (SEL != 4) process_item_ch<6, true, LPB>(a, item, lut, &trit);
             break;
         }
     }
 }
 
 hipError_t launch_encode_batch(const BatchArgs& a, const BatchLaunch& l, hipStream_t stream) {
+    (void)hipGetLastError();  // report this launch's errors only, not an earlier call's
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, a.counters);
     if (!l.split) {
-        hipLaunchKernelGGL(encode_batch_kernel<0>, dim3(unsigned(l.grid)), dim3(kPlainThreads), 0, stream, a);
+        if (l.lanes == 2)
+            hipLaunchKernelGGL((encode_batch_kernel<0, 2>), dim3(unsigned(l.grid)), dim3(kPlainThreads), 0, stream, a);
+        else
+            hipLaunchKernelGGL((encode_batch_kernel<0, 1>), dim3(unsigned(l.grid)), dim3(kPlainThreads), 0, stream, a);
         return hipGetLastError();
     }
     // 6x6 kernel first on `stream`; the 4x4 kernel on the side stream picks up
@@ -325,8 +359,11 @@ hipError_t launch_encode_batch(const BatchArgs& a, const BatchLaunch& l, hipStre
     hipError_t e;
     if ((e = hipEventRecord(l.fork, stream)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(l.side, l.fork, 0)) != hipSuccess) return e;
-    hipLaunchKernelGGL(encode_batch_kernel<6>, dim3(unsigned(l.grid)), dim3(kPlainThreads), 0, stream, a);
-    hipLaunchKernelGGL(encode_batch_kernel<4>, dim3(unsigned(l.grid4)), dim3(kPlainThreads), 0, l.side, a);
+    if (l.lanes == 2)
+        hipLaunchKernelGGL((encode_batch_kernel<6, 2>), dim3(unsigned(l.grid)), dim3(kPlainThreads), 0, stream, a);
+    else
+        hipLaunchKernelGGL((encode_batch_kernel<6, 1>), dim3(unsigned(l.grid)), dim3(kPlainThreads), 0, stream, a);
+    hipLaunchKernelGGL((encode_batch_kernel<4, 1>), dim3(unsigned(l.grid4)), dim3(kPlainThreads), 0, l.side, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = hipEventRecord(l.join, l.side)) != hipSuccess) return e;
     return hipStreamWaitEvent(stream, l.join, 0);
@@ -361,6 +398,7 @@ __global__ void __launch_bounds__(256) downsample_kernel(const uint8_t* __restri
 hipError_t launch_downsample(const uint8_t* src, int w, int h, int bpp, uint8_t* dst, int ow, int oh,
                              hipStream_t stream) {
     const int64_t total = int64_t(ow) * oh * bpp;
+    (void)hipGetLastError();  // report this launch's errors only, not an earlier call's
     hipLaunchKernelGGL(downsample_kernel, dim3(unsigned((total + 255) / 256)), dim3(256), 0, stream, src, w, h, bpp,
                        dst, ow, oh);
     return hipGetLastError();
@@ -452,6 +490,7 @@ hipError_t launch_encode_plain(const DeviceTables* d_tables, const astc_image_de
     a.dst = static_cast<uint32_t*>(dst);
     a.tables = d_tables;
     PlainTuning t = tuning ? *tuning : default_plain_tuning(d, g.nblocks, num_cus);
+    (void)hipGetLastError();  // report this launch's errors only, not an earlier call's
     if (d.block_dim != 6) t.lanes_per_block = 1;
     if (t.blocks_per_lane == 0) t.blocks_per_lane = plain_blocks_per_lane(g.nblocks, d.block_dim, t.lanes_per_block, num_cus);
     const bool alpha = d.flags & FLAG_ALPHA;

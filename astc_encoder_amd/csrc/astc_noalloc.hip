@@ -63,10 +63,9 @@ extern "C" int astc_encode_batch_noalloc(astc_ctx* ctx, const astc_image_desc* d
         if (!ctx->noalloc.scratch || !ctx->noalloc.staging || !ctx->noalloc.staged)
             return fail(ASTC_ERR_INVALID_ARG, "context has no no-alloc scratch");
         if (count > kNoAllocMaxImages) return fail(ASTC_ERR_INVALID_ARG, "more images than the no-alloc maximum (8192)");
-        const astc_tuning t = tuning ? *tuning : astc_tuning{4, 2, 1, 0};
+        const astc_tuning t = tuning ? *tuning : astc_tuning{4, 2, 1, 0, 1};
         const int bpl = t.blocks_per_lane;
-        if (!(bpl == 1 || bpl == 2 || bpl == 4 || bpl == 8) || t.workgroups_per_cu < 1 || t.workgroups_per_cu > 4 ||
-            (t.prefetch != 0 && t.prefetch != 1) || t.split_modes != 0)
+        if (!valid_batch_tuning(t) || t.split_modes != 0)
             return fail(ASTC_ERR_INVALID_ARG, "tuning out of range (the no-alloc form runs one kernel: split_modes 0)");
         hipError_t e = hipSetDevice(ctx->device);
         if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
@@ -133,6 +132,7 @@ extern "C" int astc_encode_batch_noalloc(astc_ctx* ctx, const astc_image_desc* d
         }
         if ((e = hipEventRecord(ctx->noalloc.staged, s)) != hipSuccess) return hip_fail(e, "no-alloc staging event");
         BatchLaunch l{};
+        l.lanes = batch_lanes(t);
         l.grid = std::max(1, ctx->num_cus * t.workgroups_per_cu);
         l.split = 0;
         if ((e = launch_encode_batch(a, l, s)) != hipSuccess) return hip_fail(e, "no-alloc batch launch");

@@ -209,6 +209,7 @@ int ensure_capacity(astc_server* sv, Slot& s, size_t nlev, size_t nseg, size_t n
     a.tables = sv->ctx->d_tables;
     a.prefetch = sv->cfg.tuning.prefetch;
     BatchLaunch l;
+    l.lanes = batch_lanes(sv->cfg.tuning);
     l.grid = std::max(1, sv->num_cus * sv->cfg.tuning.workgroups_per_cu);
     // the 4x4 kernel's own occupancy (five waves per SIMD), whatever the 6x6
     // kernel's: with one 6x6 workgroup per CU, three 4x4 waves share each SIMD
@@ -385,10 +386,7 @@ int astc_server_create(astc_ctx* ctx, const astc_server_config* cfg, astc_server
         if (!ctx || !cfg) return fail(ASTC_ERR_INVALID_ARG, "NULL argument");
         const astc_tuning& t = cfg->tuning;
         if (cfg->hbm_budget_bytes < kAlign || cfg->max_inflight < 1 || cfg->max_inflight > 8 ||
-            cfg->max_batch_requests < 1 ||
-            !(t.blocks_per_lane == 1 || t.blocks_per_lane == 2 || t.blocks_per_lane == 4 || t.blocks_per_lane == 8) ||
-            t.workgroups_per_cu < 1 || t.workgroups_per_cu > 4 || (t.prefetch != 0 && t.prefetch != 1) ||
-            (t.split_modes != 0 && t.split_modes != 1))
+            cfg->max_batch_requests < 1 || !valid_batch_tuning(t))
             return fail(ASTC_ERR_INVALID_ARG, "server configuration out of range");
         hipError_t e = hipSetDevice(ctx->device);
         if (e != hipSuccess) return hip_fail(e, "hipSetDevice");

@@ -78,9 +78,12 @@ class Encoder:
     """An astc_ctx on one HIP device."""
 
     def __init__(self, device=0):
+        import weakref
         self._ctx = ctypes.c_void_p()
         check(lib().astc_ctx_create(int(device), ctypes.byref(self._ctx)))
         self.device = device
+        # batches and servers made on this context: closed before it is destroyed
+        self._children = weakref.WeakSet()
 
     @property
     def handle(self):
@@ -88,6 +91,8 @@ class Encoder:
 
     def close(self):
         if self._ctx:
+            for child in list(self._children):
+                child.close()
             lib().astc_ctx_destroy(self._ctx)
             self._ctx = ctypes.c_void_p()
 
@@ -204,7 +209,7 @@ class Encoder:
         uint8, block_dim, flags), ...] into outputs [(nblocks, 16) uint8 device
         tensors] with the persistent kernel, allocating nothing (the work list
         lives in the library's fixed device array).  tuning: None or
-        (blocks_per_lane, workgroups_per_cu, prefetch)."""
+        (blocks_per_lane, workgroups_per_cu, prefetch[, lanes_per_block])."""
         n = len(images)
         if len(outputs) != n:
             raise ValueError("one output per image")
@@ -220,7 +225,8 @@ class Encoder:
                 raise ValueError(f"output {i} is too small or not contiguous")
             srcs[i] = tex.data_ptr()
             dsts[i] = out.data_ptr()
-        t = None if tuning is None else ctypes.byref(Tuning(int(tuning[0]), int(tuning[1]), int(tuning[2]), 0))
+        t = None if tuning is None else ctypes.byref(
+            Tuning(int(tuning[0]), int(tuning[1]), int(tuning[2]), 0, int(tuning[3]) if len(tuning) > 3 else 1))
         check(lib().astc_encode_batch_noalloc(self._ctx, descs, srcs, dsts, n, t, _stream_handle(stream)))
 
     def mip_chain(self, base, stream=None):
@@ -272,6 +278,7 @@ class Batch:
         self._h = ctypes.c_void_p()
         check(lib().astc_batch_create(encoder.handle, descs, srcs, dsts, n, ctypes.byref(self._h)))
         self.encoder = encoder
+        encoder._children.add(self)
         self._descs, self._srcs, self._dsts, self._n = descs, srcs, dsts, n
 
     def launch(self, stream=None):
@@ -288,8 +295,8 @@ class Batch:
         check(lib().astc_batch_autotune(self._h, _stream_handle(stream), int(reps), ctypes.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, blocks_per_lane, workgroups_per_cu, prefetch, split_modes=0):
-        t = Tuning(blocks_per_lane, workgroups_per_cu, prefetch, split_modes)
+    def set_tuning(self, blocks_per_lane, workgroups_per_cu, prefetch, split_modes=0, lanes_per_block=1):
+        t = Tuning(blocks_per_lane, workgroups_per_cu, prefetch, split_modes, lanes_per_block)
         check(lib().astc_batch_set_tuning(self._h, ctypes.byref(t)))
 
     def tuning(self):
@@ -358,11 +365,12 @@ class Server:
 
     def __init__(self, encoder, hbm_budget_bytes, max_inflight=3, max_batch_requests=64,
                  tuning=(4, 2, 1, 0)):
-        t = tuple(tuning) + (0,) * (4 - len(tuning))
+        t = tuple(tuning) + (0, 1)[len(tuning) - 3:] if len(tuning) < 5 else tuple(tuning)
         cfg = ServerConfig(int(hbm_budget_bytes), int(max_inflight), int(max_batch_requests),
-                           Tuning(t[0], t[1], t[2], t[3]))
+                           Tuning(t[0], t[1], t[2], t[3], t[4]))
         self._h = ctypes.c_void_p()
         check(lib().astc_server_create(encoder.handle, ctypes.byref(cfg), ctypes.byref(self._h)))
+        encoder._children.add(self)
 
     def run_trace(self, requests):
         """requests: list of (PackedImages, arrival_s, dst tensor or None), arrival

@@ -178,6 +178,9 @@ typedef struct astc_tuning {
     int32_t split_modes;       /* 0: one kernel for every item; 1: two kernels with their own
                                   register budgets, one for 6x6 and four-channel 4x4 items and
                                   one, on a second stream, for 4x4 RGB8 and normal-map items */
+    int32_t lanes_per_block;   /* lanes per 6x6 block: 1 (0 counts as 1), or 2: two lanes share each
+                                  6x6 block (three texel rows each) and the kernel holding 6x6 items
+                                  runs three waves per SIMD instead of two.  Same blocks either way */
 } astc_tuning;
 
 /* descs/srcs/dsts: `count` images (device pointers as for astc_encode).  The
@@ -187,8 +190,9 @@ int astc_batch_create(astc_ctx* ctx, const astc_image_desc* descs, const void* c
 /* Rebuild the work list and graph for a tuning (astc_batch_create uses defaults). */
 int astc_batch_set_tuning(astc_batch* batch, const astc_tuning* tuning);
 int astc_batch_get_tuning(const astc_batch* batch, astc_tuning* tuning);
-/* Time every candidate tuning (reps graph replays each, on `stream`, synchronous)
- * and keep the fastest; writes it to *best if not NULL. */
+/* Time every candidate tuning (blocks per lane x workgroups per CU x prefetch x
+ * split x lanes per 6x6 block; reps graph replays each, on `stream`,
+ * synchronous) and keep the fastest; writes it to *best if not NULL. */
 int astc_batch_autotune(astc_batch* batch, void* stream, int reps, astc_tuning* best);
 /* Encode every image of the batch: one graph replay on `stream`. */
 int astc_batch_launch(astc_batch* batch, void* stream);
@@ -206,7 +210,7 @@ int astc_batch_destroy(astc_batch* batch);
  * exactly the blocks astc_encode gives it.  Limits: at most 8192 images and
  * 131072 work items (chunks of 256 x blocks_per_lane blocks; images under 256
  * blocks share items); larger batches are refused with ASTC_ERR_INVALID_ARG.
- * tuning: NULL for {4, 2, 1, 0}; split_modes must be 0 (one kernel, one
+ * tuning: NULL for {4, 2, 1, 0, 1}; split_modes must be 0 (one kernel, one
  * stream).  Launches on `stream`, stream-ordered; calls on one GPU share the
  * array, so calls that overlap on the device (different streams) race, and
  * calls from several threads are serialised on the host. */
@@ -232,7 +236,7 @@ typedef struct astc_server_config {
     int64_t hbm_budget_bytes;   /* ring size (sources + blocks of admitted requests) */
     int32_t max_inflight;       /* batches on the GPU at once, 1..8 */
     int32_t max_batch_requests; /* requests per batch, >= 1 */
-    astc_tuning tuning;         /* persistent-kernel tuning (blocks_per_lane, workgroups_per_cu, prefetch) */
+    astc_tuning tuning;         /* persistent-kernel tuning */
 } astc_server_config;
 
 typedef struct astc_request {

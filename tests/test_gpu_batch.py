@@ -104,17 +104,20 @@ def test_batch_with_largest_dimensions(enc):
     b.close()
 
 
-@pytest.mark.parametrize("bpl,wpc,pf,split", [(1, 1, 0, 0), (2, 2, 1, 0), (4, 2, 0, 0), (8, 1, 1, 0), (8, 4, 1, 0),
-                                             (1, 1, 0, 1), (2, 2, 1, 1), (4, 2, 1, 1), (8, 4, 0, 1)])
-def test_every_tuning_gives_the_same_blocks(enc, bpl, wpc, pf, split):
+@pytest.mark.parametrize("bpl,wpc,pf,split,lanes", [(1, 1, 0, 0, 1), (2, 2, 1, 0, 1), (4, 2, 0, 0, 1), (8, 1, 1, 0, 1),
+                                                   (8, 4, 1, 0, 1), (1, 1, 0, 1, 1), (2, 2, 1, 1, 1), (4, 2, 1, 1, 1),
+                                                   (8, 4, 0, 1, 1), (1, 3, 0, 0, 2), (2, 2, 1, 0, 2), (4, 3, 0, 1, 2),
+                                                   (8, 1, 1, 1, 2), (8, 2, 0, 0, 2)])
+def test_every_tuning_gives_the_same_blocks(enc, bpl, wpc, pf, split, lanes):
     images = mixed_images(enc, 2, 24)
     # four-channel 4x4 images (which the split form runs in the 6x6 kernel)
     raw = torch.from_numpy(np.random.default_rng(bpl * 10 + wpc).integers(0, 256, (75, 61, 4), dtype=np.uint8)).cuda()
     images += [(raw, 4, astc.FLAG_ALPHA), (raw[:, :, :3].contiguous(), 4, 0), (raw[:9, :7], 4, astc.FLAG_SRGB)]
     want = plain_outputs(enc, images)
     b = astc.Batch(enc, images)
-    b.set_tuning(bpl, wpc, pf, split)
-    assert b.tuning() == {"blocks_per_lane": bpl, "workgroups_per_cu": wpc, "prefetch": pf, "split_modes": split}
+    b.set_tuning(bpl, wpc, pf, split, lanes)
+    assert b.tuning() == {"blocks_per_lane": bpl, "workgroups_per_cu": wpc, "prefetch": pf, "split_modes": split,
+                          "lanes_per_block": lanes}
     poison(b)
     b.launch()
     torch.cuda.synchronize()
@@ -136,7 +139,8 @@ def test_random_batches(enc, seed):
         images.append((t[:, :w], int(rng.choice([4, 6])), int(rng.integers(0, 8))))
     want = plain_outputs(enc, images)
     b = astc.Batch(enc, images)
-    tuning = (int(rng.choice([1, 2, 4, 8])), int(rng.integers(1, 3)), int(rng.integers(0, 2)), int(rng.integers(0, 2)))
+    tuning = (int(rng.choice([1, 2, 4, 8])), int(rng.integers(1, 3)), int(rng.integers(0, 2)), int(rng.integers(0, 2)),
+              int(rng.integers(1, 3)))
     print(f"case seed={seed} images={len(images)} tuning={tuning}", flush=True)
     b.set_tuning(*tuning)
     poison(b)
@@ -147,12 +151,12 @@ def test_random_batches(enc, seed):
     b.close()
 
 
-@pytest.mark.parametrize("split", [0, 1])
-def test_graph_replays_with_eager_work_between(enc, split):
+@pytest.mark.parametrize("split,lanes", [(0, 1), (1, 1), (0, 2), (1, 2)])
+def test_graph_replays_with_eager_work_between(enc, split, lanes):
     images = mixed_images(enc, 3, 16)
     want = plain_outputs(enc, images)
     b = astc.Batch(enc, images)
-    b.set_tuning(4, 2, 1, split)
+    b.set_tuning(4, 2, 1, split, lanes)
     for rep in range(4):
         for o in b.outputs:
             o.zero_()
@@ -201,7 +205,7 @@ def test_autotune_picks_a_valid_tuning(enc):
     b = astc.Batch(enc, images)
     t = b.autotune(reps=2)
     assert t["blocks_per_lane"] in (1, 2, 4, 8) and t["workgroups_per_cu"] in (1, 2)
-    assert t["split_modes"] in (0, 1)
+    assert t["split_modes"] in (0, 1) and t["lanes_per_block"] in (1, 2)
     for o in b.outputs:
         o.zero_()
     b.launch()
@@ -264,7 +268,7 @@ def test_batch_rejects_bad_images(enc):
         astc.Batch(enc, [(t, 4, 0)], outputs=[raw[1:].view(4, 16)])
 
 
-@pytest.mark.parametrize("tuning", [(3, 2, 1, 0), (4, 0, 1, 0), (4, 5, 1, 0), (4, 2, 2, 0), (4, 2, 1, 2)])
+@pytest.mark.parametrize("tuning", [(3, 2, 1, 0), (4, 0, 1, 0), (4, 5, 1, 0), (4, 2, 2, 0), (4, 2, 1, 2), (4, 2, 1, 0, 3)])
 def test_tuning_out_of_range_is_rejected_and_kept(enc, tuning):
     t = torch.zeros((8, 8, 4), dtype=torch.uint8, device="cuda")
     b = astc.Batch(enc, [(t, 4, 0)])

@@ -64,9 +64,10 @@ def test_mixed_set_full_size_batched_equals_plain_and_oracle(enc):
     b2.close()
     other = dict(tuned)
     other["split_modes"] = 1 - tuned["split_modes"]
+    other["lanes_per_block"] = 3 - tuned["lanes_per_block"]
     for tuning in (tuned, other):
         b.set_tuning(tuning["blocks_per_lane"], tuning["workgroups_per_cu"], tuning["prefetch"],
-                     tuning["split_modes"])
+                     tuning["split_modes"], tuning["lanes_per_block"])
         for o in b.outputs:
             o.fill_(0xA5)
         b.launch()

@@ -77,7 +77,7 @@ def test_alone_and_in_the_mix_give_the_same_blocks(enc, pool):
     srv.close()
 
 
-@pytest.mark.parametrize("tuning", [(8, 1, 0, 1), (2, 2, 1, 1), (4, 2, 1, 0)])
+@pytest.mark.parametrize("tuning", [(8, 1, 0, 1), (2, 2, 1, 1), (4, 2, 1, 0), (8, 1, 0, 1, 2), (4, 3, 1, 0, 2)])
 def test_every_mode_class_under_each_tuning(enc, tuning):
     """Requests of every mode class (6x6, four-channel 4x4 in linear, alpha and
     sRGB forms, tiled RGB8 4x4, normal maps), each with its own random content so
@@ -164,7 +164,8 @@ def test_oversized_request_is_rejected(enc, pool):
 
 @pytest.mark.parametrize("kw", [dict(budget=100), dict(max_inflight=0), dict(max_inflight=9),
                                 dict(max_batch_requests=0), dict(tuning=(3, 2, 1, 0)), dict(tuning=(4, 0, 1, 0)),
-                                dict(tuning=(4, 5, 1, 0)), dict(tuning=(4, 2, 2, 0)), dict(tuning=(4, 2, 1, 2))])
+                                dict(tuning=(4, 5, 1, 0)), dict(tuning=(4, 2, 2, 0)), dict(tuning=(4, 2, 1, 2)),
+                                dict(tuning=(4, 2, 1, 0, 3))])
 def test_bad_server_configs_are_rejected(enc, kw):
     args = dict(budget=1 << 20, max_inflight=2, max_batch_requests=4, tuning=(4, 2, 1, 0))
     args.update(kw)
@@ -337,9 +338,11 @@ def test_online_ring_full_and_ticket_rules(enc, pool):
     # a reserved ticket may be given back unsubmitted
     t2, _ = srv.reserve(pk)
     srv.release(t2)
+    tiny = astc.Server(enc, 1024, max_inflight=1, max_batch_requests=1)
     with pytest.raises(astc.AstcError) as e:
-        astc.Server(enc, 1024, max_inflight=1, max_batch_requests=1).reserve(pk)
+        tiny.reserve(pk)
     assert e.value.code == -6
+    tiny.close()
     # and the trace form works again on the emptied ring
     lat, st = srv.run_trace([(pk, 0.0, None)])
     assert st["requests"] == 1

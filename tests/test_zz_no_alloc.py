@@ -72,7 +72,7 @@ def test_noalloc_equals_batch_on_two_calls_second_smaller(enc):
     out1, out2 = sentinel_outputs(first), sentinel_outputs(second)
     print(f"no-alloc: {len(first)} then {len(second)} images", flush=True)
     enc.encode_batch_noalloc(first, out1)
-    enc.encode_batch_noalloc(second, out2, tuning=(2, 1, 0))
+    enc.encode_batch_noalloc(second, out2, tuning=(2, 1, 0, 2))
     torch.cuda.synchronize()
     for i, (o, w) in enumerate(zip(out1, want1)):
         np.testing.assert_array_equal(o.cpu().numpy(), w, err_msg=f"first call, image {i}")
