@@ -206,7 +206,7 @@ def gpu_psnr(enc, wl, src, blocks):
     return round(10.0 * np.log10(255.0 * 255.0 * e.numel() / sse), 4) if sse > 0 else float("inf")
 
 
-def batched_vs_plain(enc, steps, warmup, n_textures=512):
+def batched_vs_plain(enc, steps, warmup, n_textures=512, tuning=None):
     """512 seeded textures (side 2^k, k in 6..12, square or 2:1; 60% sinusoid RGB8
     4x4 / 25% noise RGBA8 sRGB 6x6 / 15% normal RG8 4x4), each with a full mip
     chain generated on the device: one autotuned persistent-kernel graph replay
@@ -224,7 +224,11 @@ def batched_vs_plain(enc, steps, warmup, n_textures=512):
     torch.cuda.synchronize()
     gen_s = time.perf_counter() - t0
     b = astc.Batch(enc, images)
-    tuning = b.autotune(reps=10)
+    if tuning is None:
+        tuning = b.autotune(reps=10)
+    else:
+        b.set_tuning(*tuning)
+        tuning = b.tuning()
     info = b.info()
     stream = torch.cuda.current_stream()
 
@@ -428,6 +432,9 @@ def main():
     ap.add_argument("--plain-tuning", default=None,
                     help="lanes_per_block,blocks_per_lane,prefetch[,flat_alpha] for the headline's plain launches "
                          "(default: the library's choice)")
+    ap.add_argument("--batched-tuning", default=None,
+                    help="blocks_per_lane,workgroups_per_cu,prefetch,split_modes,lanes_per_block for the mixed set "
+                         "(default: the autotuner's choice)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run 3 device copies of 256 MiB (a known-size kernel for scaling HBM counters)")
     args = ap.parse_args()
@@ -464,7 +471,8 @@ def main():
         extras["opaque_rgba8_6x6_autotuned"] = opaque_autotuned(enc, max(10, args.steps), args.warmup)
         extras["host_resident_headline"] = time_host_resident(enc, head, 20)
         if not args.no_batched:
-            extras["mixed_textures_batched_vs_plain"] = batched_vs_plain(enc, max(5, args.steps // 5), 2)
+            fixed = tuple(int(v) for v in args.batched_tuning.split(",")) if args.batched_tuning else None
+            extras["mixed_textures_batched_vs_plain"] = batched_vs_plain(enc, max(5, args.steps // 5), 2, tuning=fixed)
         if not args.no_serving:
             extras["serving"] = serving_sweep(enc)
     head_tuning = tuple(int(v) for v in args.plain_tuning.split(",")) if args.plain_tuning else None
