@@ -305,7 +305,8 @@ def serving_sweep(enc, rates=(1000, 2000, 5000, 10000, 20000, 50000), seconds=0.
     srv = astc.Server(enc, 64 << 30, max_inflight=3, max_batch_requests=256, tuning=SATURATION_TUNING)
     srv.run_trace([(pool[i % pool_size], 0.0, None) for i in range(4 * pool_size)])
     sat = saturation(srv, pool, rng)
-    sat["tuning"] = dict(zip(("blocks_per_lane", "workgroups_per_cu", "prefetch", "split_modes"), SATURATION_TUNING))
+    sat["tuning"] = dict(zip(("blocks_per_lane", "workgroups_per_cu", "prefetch", "split_modes", "lanes_per_block"),
+                             SATURATION_TUNING + (1,)))
     srv.close()
     return {"workload": f"Poisson arrivals, {pool_size} mip-chained textures of the mixed distribution (seed 1)",
             "hbm_budget_gb": 64, "max_inflight_batches": 3, "sweep": results, "saturation": sat}
@@ -489,8 +490,11 @@ def main():
             "flops_per_block": FLOPS_PER_BLOCK[head["flops_key"]]}
     # the roofline that binds is the one the kernel comes closest to
     bound = dict(hbm if hbm["frac"] >= valu["frac"] else valu)
-    kernel = ("encode_plain_pair_kernel<true, true, 4>" if head_tuning and head_tuning[0] == 2
-              else "encode_plain_kernel<6, true, true, 4, 0>")
+    if head_tuning and head_tuning[0] == 2:
+        kernel = "encode_plain_pair_kernel<true, true, 4>"
+    else:
+        flat = bool(head_tuning and len(head_tuning) > 3 and head_tuning[3])
+        kernel = f"encode_plain_kernel<6, true, true, 4, 0, {'true' if flat else 'false'}>"
     bound.update({"kernel": kernel, "plain_tuning": head_tuning or "library default",
                   "kernel_ms": round(r["kernel_ms"], 4),
                   "step_event_ms": round(r["step_event_ms"], 4), "algorithmic_bytes": algo_bytes,
