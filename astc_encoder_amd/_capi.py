@@ -49,11 +49,11 @@ class Tuning(ctypes.Structure):
 class PlainTuning(ctypes.Structure):
     """astc_plain_tuning"""
     _fields_ = [("lanes_per_block", ctypes.c_int32), ("blocks_per_lane", ctypes.c_int32),
-                ("prefetch", ctypes.c_int32), ("flat_alpha", ctypes.c_int32)]
+                ("prefetch", ctypes.c_int32), ("flat_alpha", ctypes.c_int32), ("packed_texels", ctypes.c_int32)]
 
     def as_dict(self):
         return {"lanes_per_block": self.lanes_per_block, "blocks_per_lane": self.blocks_per_lane,
-                "prefetch": self.prefetch, "flat_alpha": self.flat_alpha}
+                "prefetch": self.prefetch, "flat_alpha": self.flat_alpha, "packed_texels": self.packed_texels}
 
 
 class ServerConfig(ctypes.Structure):

@@ -305,6 +305,100 @@ __device__ __forceinline__ void unpack_block(const uint32_t p[NR * Layout<DIM, C
     }
 }
 
+// A block's texels kept as the fetched words (Layout<DIM, CH>) and converted
+// at every read through the LDS tables, as unpack_block converts them (the
+// same floats): a lane holds 1-4 bytes per texel instead of four floats.
+// encode_block reads the texels in four passes (sums, covariance, extreme
+// projections, weight projections); texel_pass between them makes each pass
+// convert again instead of keeping the previous pass's floats live.
+template <int DIM, int CH>
+struct PackedTexels {
+    uint32_t p[Layout<DIM, CH>::WORDS];
+    const float* colour;  // colour-channel table (sRGB or UNORM)
+    const float* unorm;   // alpha table
+    struct Texel {
+        float v[4];
+        __device__ __forceinline__ operator const float*() const { return v; }
+    };
+    __device__ __forceinline__ Texel operator[](int k) const {
+        Texel t;
+        if (CH == CH_ALL) {
+            const uint32_t w = p[k];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) t.v[c] = colour[(w >> (8 * c)) & 0xFFu];
+            t.v[3] = unorm[w >> 24];
+        } else {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) t.v[c] = c < CH ? colour[layout_byte<DIM, CH>(p, k, c)] : 255.0f;
+        }
+        return t;
+    }
+};
+
+// Between two passes over a block's texels: nothing for converted floats; for
+// packed words a compiler memory barrier, so that the next pass reloads the
+// table entries rather than reusing (and keeping live) the last pass's.
+template <typename TX>
+__device__ __forceinline__ void texel_pass(const TX&) {}
+template <int DIM, int CH>
+__device__ __forceinline__ void texel_pass(const PackedTexels<DIM, CH>&) {
+    asm volatile("" ::: "memory");
+}
+
+// After texel k of a pass, for packed words, at the end of each texel row: the
+// same barrier, with the pass's running values v[0..NV) (sums, extremes,
+// projections) taken through it, so that each row's table reads are consumed
+// before the next row's are issued (without it a pass's reads are all issued,
+// and held, at once).
+template <int NV, typename TX>
+__device__ __forceinline__ void texel_step(const TX&, int, float*) {}
+template <int NV, int DIM, int CH>
+__device__ __forceinline__ void texel_step(const PackedTexels<DIM, CH>&, int k, float* v) {
+    if (k % DIM == DIM - 1) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) asm volatile("" : "+v"(v[i]));
+        asm volatile("" ::: "memory");
+    }
+}
+
+// Per-channel sums of the first NC channels over texels [0, H) and [H, N),
+// added (one chain of additions per channel and range, in texel order).
+template <int NC, int H, int N>
+__device__ __forceinline__ void block_sums(const float (&x)[N][4], float sum[4]) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        float r = x[0][c];
+#pragma unroll
+        for (int k = 1; k < H; ++k) r = r + x[k][c];
+        if (H < N) {
+            float r2 = x[H][c];
+#pragma unroll
+            for (int k = H + 1; k < N; ++k) r2 = r2 + x[k][c];
+            r = r + r2;
+        }
+        sum[c] = r;
+    }
+}
+// The same sums texel by texel (each texel converted once per pass).
+template <int NC, int H, int N, int DIM, int CH>
+__device__ __forceinline__ void block_sums(const PackedTexels<DIM, CH>& x, float sum[4]) {
+    float r[4], r2[4];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const typename PackedTexels<DIM, CH>::Texel t = x[k];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            if (k == 0) r[c] = t.v[c];
+            else if (k < H) r[c] = r[c] + t.v[c];
+            else if (k == H) r2[c] = t.v[c];
+            else r2[c] = r2[c] + t.v[c];
+        }
+        texel_step<NC>(x, k, k < H ? r : r2);
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) sum[c] = H < N ? r[c] + r2[c] : r[c];
+}
+
 // ---------------------------------------------------------------------------
 // Integer stage: endpoint bytes + weight trits -> 128-bit block
 // (ASTC_Encode.hlsl:400-444, 475-508; ASTC_IntegerSequenceEncoding.hlsl:142-176, 207-276)
@@ -462,9 +556,8 @@ __device__ __forceinline__ void covariance_matrix(const float cov[NC * (NC + 1) 
 // Sums of products of the differences from the mean over texels [K0, K1) of x:
 // one fused multiply-add chain per upper-triangle entry, starting with a plain
 // product.
-template <int NC, int K0, int K1, int NX>
-__device__ __forceinline__ void covariance_chain(const float x[NX][4], const float mean[4],
-                                                 float cov[NC * (NC + 1) / 2]) {
+template <int NC, int K0, int K1, typename TX>
+__device__ __forceinline__ void covariance_chain(const TX& x, const float mean[4], float cov[NC * (NC + 1) / 2]) {
 #pragma unroll
     for (int k = K0; k < K1; ++k) {
         float d[4];
@@ -475,6 +568,7 @@ __device__ __forceinline__ void covariance_chain(const float x[NX][4], const flo
         for (int a = 0; a < NC; ++a)
 #pragma unroll
             for (int b2 = a; b2 < NC; ++b2, ++p) cov[p] = (k == K0) ? d[a] * d[b2] : fma_(d[a], d[b2], cov[p]);
+        texel_step<NC * (NC + 1) / 2>(x, k, cov);
     }
 }
 
@@ -597,10 +691,10 @@ __device__ __forceinline__ bool flat_alpha(const uint32_t words[N], const float*
 // RESCALE: the power iteration's rescale interval, 3 or 1 (every product); both
 // give the same blocks (see principal_axis).  alpha: the constant alpha of a
 // four-channel block encoded by the three-channel path (NC = 3; see
-// flat_alpha), else 255.
-template <int DIM, bool ALPHA, int NC = 4, int METHOD = METHOD_PCA, int RESCALE = 3>
-__device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], const PackTables* pack,
-                                                 float alpha = 255.0f) {
+// flat_alpha), else 255.  x: the texels, float x[DIM * DIM][4] or
+// PackedTexels<DIM, CH> (x[k][c] is texel k's channel c, times 255).
+template <int DIM, bool ALPHA, int NC = 4, int METHOD = METHOD_PCA, int RESCALE = 3, typename TX>
+__device__ __forceinline__ Block128 encode_block(const TX& x, const PackTables* pack, float alpha = 255.0f) {
     static_assert(NC >= 2 && NC <= 4, "two to four active channels");
     constexpr int N = DIM * DIM;
     constexpr int H = SumHalf<DIM>::value;
@@ -608,19 +702,13 @@ __device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], co
 
     // mean (principal_component_analysis :142-147)
     float mean[4] = {255.0f, 255.0f, 255.0f, 255.0f};
+    {
+        float sum[4];
+        block_sums<NC, H, N>(x, sum);
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-        float r = x[0][c];
-#pragma unroll
-        for (int k = 1; k < H; ++k) r = r + x[k][c];
-        if (H < N) {
-            float r2 = x[H][c];
-#pragma unroll
-            for (int k = H + 1; k < N; ++k) r2 = r2 + x[k][c];
-            r = r + r2;
-        }
-        mean[c] = r * rcp_n;
+        for (int c = 0; c < NC; ++c) mean[c] = sum[c] * rcp_n;
     }
+    texel_pass(x);
 
     float v[4];
     if (METHOD == METHOD_MAX_ACCUMULATION) {
@@ -669,10 +757,10 @@ __device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], co
         // covariance (:149-162): upper triangle of the active channels
         constexpr int NP = NC * (NC + 1) / 2;
         float cov[NP];
-        covariance_chain<NC, 0, H, N>(x, mean, cov);
+        covariance_chain<NC, 0, H>(x, mean, cov);
         if (H < N) {
             float cov2[NP];
-            covariance_chain<NC, H, N, N>(x, mean, cov2);
+            covariance_chain<NC, H, N>(x, mean, cov2);
 #pragma unroll
             for (int p = 0; p < NP; ++p) cov[p] = cov[p] + cov2[p];
         }
@@ -680,17 +768,21 @@ __device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], co
         covariance_matrix<NC, N>(cov, m);
         principal_axis<NC, RESCALE>(m, v);
     }
+    texel_pass(x);
 
     // find_min_max (:108-137): t_k = dot(texel_k - mean, v), taken as
     // dot(texel_k, v) - dot(mean, v); rounding is monotonic, so the minimum and
     // maximum of t_k are the extreme projections minus dot(mean, v).
-    float lo = 1e31f, hi = -1e31f;
+    float ext[2] = {1e31f, -1e31f};  // lowest and highest projection
 #pragma unroll
     for (int k = 0; k < N; ++k) {
         const float t = dotn<NC>(x[k], v);
-        lo = fminf(lo, t);
-        hi = fmaxf(hi, t);
+        ext[0] = fminf(ext[0], t);
+        ext[1] = fmaxf(ext[1], t);
+        texel_step<2>(x, k, ext);
     }
+    const float lo = ext[0], hi = ext[1];
+    texel_pass(x);
     float e0[4], e1[4];
     uint32_t e0q[4], e1q[4];
     const bool swap = endpoints<ALPHA, NC>(v, lo, hi, mean, e0, e1, e0q, e1q, alpha);
@@ -703,11 +795,17 @@ __device__ __forceinline__ Block128 encode_block(const float x[DIM * DIM][4], co
     if (DIM == 6) {
         float pt[N];
 #pragma unroll
-        for (int j = 0; j < N; ++j) pt[j] = dotn<KC>(k, x[j]);
+        for (int j = 0; j < N; ++j) {
+            pt[j] = dotn<KC>(k, x[j]);
+            texel_step<DIM>(x, j, pt + (j / DIM) * DIM);
+        }
         grid6_samples<0, 16, 0, N>(pt, proj);
     } else {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) proj[i] = dotn<KC>(k, x[i]);
+        for (int i = 0; i < 16; ++i) {
+            proj[i] = dotn<KC>(k, x[i]);
+            texel_step<DIM>(x, i, proj + (i / DIM) * DIM);
+        }
     }
     float minw = 1e31f, maxw = -1e31f;
 #pragma unroll
@@ -757,7 +855,7 @@ __device__ __forceinline__ Block128 encode_block_pair(const float x[18][4], int 
     }
     constexpr int NP = NC * (NC + 1) / 2;
     float cov[NP];
-    covariance_chain<NC, 0, NH, NH>(x, mean, cov);
+    covariance_chain<NC, 0, NH>(x, mean, cov);
 #pragma unroll
     for (int p = 0; p < NP; ++p) cov[p] = cov[p] + pair_other(cov[p]);
     float m[4][4];

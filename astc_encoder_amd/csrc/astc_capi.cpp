@@ -165,14 +165,17 @@ int astc_plain_autotune(astc_ctx* ctx, const astc_image_desc* desc, const void* 
         // the flat-alpha test matters only where the four-channel encoder runs
         const bool four = desc->src_format == ASTC_FMT_RGBA8 && !(desc->flags & ASTC_FLAG_NORMAL) &&
                           !(desc->flags & ASTC_FLAG_MAX_ACCUMULATION);
-        for (int lanes : {1, 2}) {
-            if (lanes == 2 && (desc->block_dim != 6 || (desc->flags & ASTC_FLAG_MAX_ACCUMULATION))) continue;
+        const bool maxacc = desc->flags & ASTC_FLAG_MAX_ACCUMULATION;
+        // lanes 0 stands for one lane with packed texels
+        for (int lanes : {1, 2, 0}) {
+            if (lanes != 1 && (maxacc || (lanes == 2 && desc->block_dim != 6))) continue;
             for (int bpl : {1, 2, 4, 8})
                 for (int pf : {0, 1})
                     for (int flat : {0, 1}) {
                     if (rc != ASTC_OK) break;
-                    if (flat && (lanes == 2 || !four)) continue;
-                    const astc_plain_tuning t{lanes, bpl, pf, flat};
+                    if (flat && (lanes != 1 || !four)) continue;
+                    if (lanes == 0 && pf) continue;
+                    const astc_plain_tuning t{lanes == 0 ? 1 : lanes, bpl, pf, flat, lanes == 0 ? 1 : 0};
                     if ((rc = encode_one(ctx, desc, src_dev, dst_dev, stream, &t)) != ASTC_OK) break;  // warm-up
                     if ((e = hipEventRecord(t0, s)) != hipSuccess) {
                         rc = hip_fail(e, "hipEventRecord");
@@ -217,12 +220,15 @@ int encode_one(astc_ctx* ctx, const astc_image_desc* desc, const void* src_dev, 
         const int bpl = tuning->blocks_per_lane;
         if ((tuning->lanes_per_block != 1 && tuning->lanes_per_block != 2) ||
             !(bpl == 0 || bpl == 1 || bpl == 2 || bpl == 4 || bpl == 8) ||
-            (tuning->prefetch != 0 && tuning->prefetch != 1) || (tuning->flat_alpha != 0 && tuning->flat_alpha != 1))
+            (tuning->prefetch != 0 && tuning->prefetch != 1) || (tuning->flat_alpha != 0 && tuning->flat_alpha != 1) ||
+            (tuning->packed_texels != 0 && tuning->packed_texels != 1) ||
+            (tuning->packed_texels && (tuning->lanes_per_block != 1 || tuning->prefetch || tuning->flat_alpha)))
             return fail(ASTC_ERR_INVALID_ARG, "plain tuning out of range");
         pt.lanes_per_block = tuning->lanes_per_block;
         pt.blocks_per_lane = bpl;
         pt.prefetch = tuning->prefetch;
         pt.flat_alpha = tuning->flat_alpha;
+        pt.packed_texels = tuning->packed_texels;
     }
     ImageGeom g;
     int rc = check_desc(desc, &g);

@@ -184,6 +184,7 @@ struct PlainTuning {
     int blocks_per_lane;  // blocks per lane per workgroup: 1, 2, 4, 8, or 0 for the library's choice
     int prefetch;         // 1: the next block's loads are issued before the current block is encoded
     int flat_alpha;       // 1: one-lane four-channel encodes test each wave for a flat alpha
+    int packed_texels;    // 1: one-lane encodes hold the fetched words and convert them in every pass
 };
 hipError_t launch_encode_plain(const DeviceTables* d_tables, const astc_image_desc& d, const ImageGeom& g,
                                const void* src, void* dst, hipStream_t stream, int num_cus,

@@ -13,7 +13,9 @@
 // of the SIMD cover the load (measured faster than issuing the next block's
 // loads first, which stays a tuning); the 6x6 encoder holds 144 texel floats
 // per lane and runs two waves per SIMD.  encode_plain_pair_kernel is the
-// two-lanes-per-block 6x6 form (astc_plain_tuning.lanes_per_block = 2).
+// two-lanes-per-block 6x6 form (astc_plain_tuning.lanes_per_block = 2), and
+// encode_plain_packed_kernel the form that holds the fetched bytes instead of
+// floats (astc_plain_tuning.packed_texels = 1).
 #include "astc_internal.h"
 
 namespace astc {
@@ -144,6 +146,42 @@ __attribute__((amdgpu_waves_per_eu(PlainWaves<DIM>::value))) encode_plain_kernel
     const int first = blockIdx.x * chunk;
     const int count = min(chunk, a.nblocks - first);
     encode_chunk<DIM, ALPHA, SRGB ? 1 : 0, CH, METHOD, FLAT>(a.f, a.dst, first, count, prefetch != 0, lut, &trit);
+}
+
+// ---------------------------------------------------------------------------
+// One lane per block with the texels held as the fetched words (PackedTexels,
+// astc_plain_tuning.packed_texels = 1): each pass of the encode converts them
+// again through the LDS tables, which trades VALU and LDS instructions for
+// registers: a 6x6 four-channel block needs ~170 VGPRs instead of 218 (three
+// waves per SIMD instead of two), a 4x4 one ~96 (five instead of four).  The
+// words are live through the encode, so the next block is loaded after the
+// store (no prefetch).  Principal component encodes only.
+// ---------------------------------------------------------------------------
+template <int DIM>
+struct PackedWaves { static constexpr int value = DIM == 4 ? 5 : 3; };
+
+template <int DIM, bool ALPHA, bool SRGB, int CH>
+__global__ void __launch_bounds__(kPlainThreads) __attribute__((amdgpu_waves_per_eu(PackedWaves<DIM>::value)))
+encode_plain_packed_kernel(PlainArgs a, int bpl) {
+    __shared__ float lut[512];
+    __shared__ PackTables trit;
+    stage_tables(a.tables, lut, &trit);
+    __syncthreads();
+    const int chunk = kPlainThreads * bpl;
+    const int first = blockIdx.x * chunk;
+    const int count = min(chunk, a.nblocks - first);
+    if (int(threadIdx.x) >= count) return;
+    PackedTexels<DIM, CH> x;
+    x.colour = SRGB ? lut : lut + 256;
+    x.unorm = lut + 256;
+    load_block_id<DIM, CH>(a.f, first + threadIdx.x, x.p);
+#pragma unroll 1
+    for (int o = threadIdx.x;; o += kPlainThreads) {
+        const bool has_next = o + kPlainThreads < count;
+        store_block(a.dst, first + o, encode_block<DIM, ALPHA, CH, METHOD_PCA, 3>(x, &trit));
+        if (!has_next) break;
+        load_block_id<DIM, CH>(a.f, first + o + kPlainThreads, x.p);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -420,6 +458,15 @@ static void launch_plain_ch(const PlainArgs& a, int64_t nblocks, const PlainTuni
     }
     const int chunk = kPlainThreads * t.blocks_per_lane;
     const dim3 grid(unsigned((nblocks + chunk - 1) / chunk));
+    if (METHOD == METHOD_PCA && t.packed_texels) {
+        if (srgb)
+            hipLaunchKernelGGL((encode_plain_packed_kernel<DIM, ALPHA, true, CH>), grid, dim3(kPlainThreads), 0, stream,
+                               a, t.blocks_per_lane);
+        else
+            hipLaunchKernelGGL((encode_plain_packed_kernel<DIM, ALPHA, false, CH>), grid, dim3(kPlainThreads), 0,
+                               stream, a, t.blocks_per_lane);
+        return;
+    }
     constexpr bool kFlatCapable = CH == CH_ALL && METHOD == METHOD_PCA;
     if (kFlatCapable && t.flat_alpha) {
         if (srgb)
@@ -470,6 +517,7 @@ PlainTuning default_plain_tuning(const astc_image_desc& d, int64_t nblocks, int 
     t.lanes_per_block = 1;
     t.prefetch = 0;
     t.flat_alpha = 0;
+    t.packed_texels = 0;
     t.blocks_per_lane = plain_blocks_per_lane(nblocks, d.block_dim, t.lanes_per_block, num_cus);
     return t;
 }

@@ -140,7 +140,7 @@ class Encoder:
     def encode(self, texture, block_dim=4, flags=0, out=None, stream=None, tuning=None):
         """Encode a device texture (H, W, C) uint8 tensor, C in {4, 3, 2}, whose
         rows hold contiguous texels (any row pitch).  tuning: None (the library's
-        default) or (lanes_per_block, blocks_per_lane, prefetch[, flat_alpha]).
+        default) or (lanes_per_block, blocks_per_lane, prefetch[, flat_alpha[, packed_texels]]).
 
         Returns a (nblocks, 16) uint8 device tensor in row-major block order."""
         self._texture_desc(texture, block_dim, flags)
@@ -150,7 +150,7 @@ class Encoder:
         if tuning is None:
             self.encode_raw(desc, texture.data_ptr(), out.data_ptr(), stream)
         else:
-            t = PlainTuning(int(tuning[0]), int(tuning[1]), int(tuning[2]), int(tuning[3]) if len(tuning) > 3 else 0)
+            t = PlainTuning(*(int(v) for v in tuple(tuning) + (0,) * (5 - len(tuning))))
             check(lib().astc_encode_tuned(self._ctx, ctypes.byref(desc), ctypes.c_void_p(texture.data_ptr()),
                                           ctypes.c_void_p(out.data_ptr()), ctypes.byref(t), _stream_handle(stream)))
         return out

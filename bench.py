@@ -89,7 +89,7 @@ def time_plain(enc, wl, steps, warmup, tuning=None):
         fn, args = astc.lib().astc_encode, (ctx, dref, sp, dp, st)
     else:
         from astc_encoder_amd._capi import PlainTuning
-        t = PlainTuning(*(tuple(tuning) + (0,) * (4 - len(tuning))))
+        t = PlainTuning(*(tuple(tuning) + (0,) * (5 - len(tuning))))
         fn, args = astc.lib().astc_encode_tuned, (ctx, dref, sp, dp, ctypes.byref(t), st)
 
     def launch():
@@ -431,7 +431,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the mixed-texture batched vs plain run")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving sweep")
     ap.add_argument("--plain-tuning", default=None,
-                    help="lanes_per_block,blocks_per_lane,prefetch[,flat_alpha] for the headline's plain launches "
+                    help="lanes_per_block,blocks_per_lane,prefetch[,flat_alpha[,packed_texels]] for the headline's plain launches "
                          "(default: the library's choice)")
     ap.add_argument("--batched-tuning", default=None,
                     help="blocks_per_lane,workgroups_per_cu,prefetch,split_modes,lanes_per_block for the mixed set "
@@ -492,6 +492,8 @@ def main():
     bound = dict(hbm if hbm["frac"] >= valu["frac"] else valu)
     if head_tuning and head_tuning[0] == 2:
         kernel = "encode_plain_pair_kernel<true, true, 4>"
+    elif head_tuning and len(head_tuning) > 4 and head_tuning[4]:
+        kernel = "encode_plain_packed_kernel<6, true, true, 4>"
     else:
         flat = bool(head_tuning and len(head_tuning) > 3 and head_tuning[3])
         kernel = f"encode_plain_kernel<6, true, true, 4, 0, {'true' if flat else 'false'}>"

@@ -104,12 +104,18 @@ int astc_encode(astc_ctx* ctx, const astc_image_desc* desc, const void* src_dev,
  * four-channel blocks (RGBA8 sources, one lane per block) for an alpha that is
  * the same integer-valued byte in every texel of every block, and encodes such
  * waves with the three-channel encoder (faster on opaque textures, a little
- * slower where few waves qualify).  Every tuning gives the same blocks. */
+ * slower where few waves qualify).  packed_texels: 1 keeps each block's texels
+ * as the fetched bytes and converts them again in every pass of the encode
+ * (fewer registers, more waves per SIMD, more instructions); it needs
+ * lanes_per_block 1, prefetch 0 and flat_alpha 0, and applies to the principal
+ * component encodes (the max-accumulation search runs the default kernel).
+ * Every tuning gives the same blocks. */
 typedef struct astc_plain_tuning {
     int32_t lanes_per_block;
     int32_t blocks_per_lane;
     int32_t prefetch;
     int32_t flat_alpha;
+    int32_t packed_texels;
 } astc_plain_tuning;
 
 /* astc_encode with an explicit plain-kernel tuning (astc_encode uses the
