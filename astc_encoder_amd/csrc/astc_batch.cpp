@@ -1,8 +1,10 @@
 // astc_batch.cpp -- batched persistent encoding: work-list planning, hipGraph
 // capture and the autotuner (include/astc_mi355x.h, astc_batch_*).
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <new>
+#include <stdexcept>
 #include <vector>
 
 #include "astc_internal.h"
@@ -43,6 +45,26 @@ namespace astc {
 // Mode code of an image that is not in the work list (no code the planner visits).
 constexpr int kNoMode = -1;
 
+// Image index bits of a planner sort key.
+constexpr int kKeyIndexBits = 27;
+
+// Sort n unsigned keys (distinct, so the result is one order) in place with a
+// least-significant-digit radix sort over 8-bit digits; tmp holds n keys.  A
+// digit that is the same in every key is skipped.
+void radix_sort_keys(uint64_t* keys, uint64_t* tmp, int n) {
+    uint64_t* src = keys;
+    uint64_t* dst = tmp;
+    for (int shift = 0; shift < 64; shift += 8) {
+        int count[257] = {0};
+        for (int i = 0; i < n; ++i) ++count[((src[i] >> shift) & 0xFF) + 1];
+        if (n == 0 || count[((src[0] >> shift) & 0xFF) + 1] == n) continue;
+        for (int d = 0; d < 256; ++d) count[d + 1] += count[d];
+        for (int i = 0; i < n; ++i) dst[count[(src[i] >> shift) & 0xFF]++] = src[i];
+        std::swap(src, dst);
+    }
+    if (src != keys) std::memcpy(keys, src, size_t(n) * sizeof(uint64_t));
+}
+
 // Position of each mode code in the planner's visiting order: most expensive
 // first (block size, then active channels, alpha, sRGB), ties by code.
 int mode_position(int mode) {
@@ -60,11 +82,13 @@ int mode_position(int mode) {
 // 4x4 reduced-channel mode in that order (4x4 four-channel items rank above
 // them and go with the 6x6 items).  Codes with both class bits set are not
 // modes (no image has them); images with a mode outside 0..MODE_COUNT-1 are
-// left out.  Writes into the caller's arrays (order: nlevels ints of room) and
-// allocates nothing; returns false if segs or items would exceed their capacity.
-bool plan_work_into(const BatchLevel* levels, const int* modes, int nlevels, int blocks_per_lane, int* order,
+// left out.  Writes into the caller's arrays (keys: 2 nlevels of room) and
+// allocates nothing; returns false if segs or items would exceed their capacity
+// or nlevels is 2^27 or more.
+bool plan_work_into(const BatchLevel* levels, const int* modes, int nlevels, int blocks_per_lane, uint64_t* keys,
                     BatchSeg* segs, int seg_cap, BatchItem* items, int item_cap, int* nseg, int* nitem,
                     int* first_light) {
+    if (nlevels < 0 || nlevels >= (1 << kKeyIndexBits)) return false;
     int pos[MODE_COUNT];
     int light_pos = MODE_COUNT;
     for (int m = 0; m < MODE_COUNT; ++m) {
@@ -72,15 +96,16 @@ bool plan_work_into(const BatchLevel* levels, const int* modes, int nlevels, int
         const int ch = m & MODE_CH_MASK;
         if (!(m & 2) && (ch == MODE_CH_RGB8 || ch == MODE_CH_NORMAL)) light_pos = std::min(light_pos, pos[m]);
     }
+    // visiting order: mode position, then blocks (large first), then index, as
+    // one unsigned key per image (position | complement of the block count |
+    // index), sorted by a least-significant-digit radix sort
     int n = 0;
     for (int i = 0; i < nlevels; ++i)
-        if (modes[i] >= 0 && modes[i] < MODE_COUNT) order[n++] = i;
-    // a total order (index breaks ties), so the in-place sort is deterministic
-    std::sort(order, order + n, [&](int a, int b) {
-        if (pos[modes[a]] != pos[modes[b]]) return pos[modes[a]] < pos[modes[b]];
-        if (levels[a].nblocks != levels[b].nblocks) return levels[a].nblocks > levels[b].nblocks;
-        return a < b;
-    });
+        if (modes[i] >= 0 && modes[i] < MODE_COUNT)
+            keys[n++] = uint64_t(pos[modes[i]]) << (kKeyIndexBits + 31) |
+                        uint64_t(0x7FFFFFFF - std::max(0, levels[i].nblocks)) << kKeyIndexBits | uint64_t(i);
+    radix_sort_keys(keys, keys + n, n);
+    const uint64_t index_mask = (uint64_t(1) << kKeyIndexBits) - 1;
     const int chunk = kPlainThreads * blocks_per_lane;
     int ns = 0, ni = 0;
     *first_light = -1;
@@ -96,7 +121,7 @@ bool plan_work_into(const BatchLevel* levels, const int* modes, int nlevels, int
         return true;
     };
     for (int k = 0; k < n; ++k) {
-        const int li = order[k], mode = modes[li];
+        const int li = int(keys[k] & index_mask), mode = modes[li];
         if (mode != cur_mode) {
             if (!flush()) return false;
             cur_mode = mode;
@@ -135,12 +160,13 @@ void plan_work(const std::vector<BatchLevel>& levels, const std::vector<int>& mo
     const int64_t chunk = int64_t(kPlainThreads) * blocks_per_lane;
     int64_t cap = 1;
     for (const BatchLevel& L : levels) cap += (L.nblocks + chunk - 1) / chunk;
-    std::vector<int> order(levels.size());
+    std::vector<uint64_t> keys(2 * levels.size());
     segs->resize(size_t(cap));
     items->resize(size_t(cap));
     int ns = 0, ni = 0;
-    plan_work_into(levels.data(), modes.data(), (int)levels.size(), blocks_per_lane, order.data(), segs->data(),
-                   (int)cap, items->data(), (int)cap, &ns, &ni, first_light);
+    if (!plan_work_into(levels.data(), modes.data(), (int)levels.size(), blocks_per_lane, keys.data(), segs->data(),
+                        (int)cap, items->data(), (int)cap, &ns, &ni, first_light))
+        throw std::length_error("work list: too many images");
     segs->resize(size_t(ns));
     items->resize(size_t(ni));
 }

@@ -24,7 +24,7 @@ struct NoAllocState {
     NoAllocPlan* staging = nullptr;
     hipEvent_t staged = nullptr;
     std::mutex mu;
-    int order[8192];
+    uint64_t keys[2 * 8192];
     int modes[8192];
 };
 }  // namespace astc
@@ -149,7 +149,7 @@ inline int mode_cost_rank(int mode) {
 // Work-list planner (astc_batch.cpp) into caller arrays, allocation-free;
 // false if the segments or items exceed their capacity.  plan_work sizes
 // vectors for it.
-bool plan_work_into(const BatchLevel* levels, const int* modes, int nlevels, int blocks_per_lane, int* order,
+bool plan_work_into(const BatchLevel* levels, const int* modes, int nlevels, int blocks_per_lane, uint64_t* keys,
                     BatchSeg* segs, int seg_cap, BatchItem* items, int item_cap, int* nseg, int* nitem,
                     int* first_light);
 
@@ -165,7 +165,9 @@ struct NoAllocPlan {
 };
 constexpr size_t kNoAllocScratchBytes = (sizeof(NoAllocPlan) + 255) / 256 * 256;
 static_assert(sizeof(BatchLevel) == 64 && sizeof(BatchSeg) == 16 && sizeof(BatchItem) == 16, "work-list layout");
-static_assert(sizeof(NoAllocState::order) / sizeof(int) == kNoAllocMaxImages, "no-alloc host arrays");
+static_assert(sizeof(NoAllocState::keys) / sizeof(uint64_t) == 2 * kNoAllocMaxImages &&
+                  sizeof(NoAllocState::modes) / sizeof(int) == kNoAllocMaxImages,
+              "no-alloc host arrays");
 // Set up / release the no-alloc state of a context (astc_noalloc.hip), with the
 // context's device current.
 int noalloc_setup(astc_ctx* ctx);

@@ -103,7 +103,7 @@ extern "C" int astc_encode_batch_noalloc(astc_ctx* ctx, const astc_image_desc* d
             }
         }
         int nseg = 0, nitem = 0, first_light = 0;
-        if (!plan_work_into(P.levels, modes, count, bpl, ctx->noalloc.order, P.segs, kNoAllocMaxItems, P.items,
+        if (!plan_work_into(P.levels, modes, count, bpl, ctx->noalloc.keys, P.segs, kNoAllocMaxItems, P.items,
                             kNoAllocMaxItems, &nseg, &nitem, &first_light))
             return fail(ASTC_ERR_INVALID_ARG, "the batch needs more work items than the no-alloc maximum (131072)");
         P.scalars[0] = nitem;
