@@ -191,7 +191,7 @@ struct PlainTuning {
 hipError_t launch_encode_plain(const DeviceTables* d_tables, const astc_image_desc& d, const ImageGeom& g,
                                const void* src, void* dst, hipStream_t stream, int num_cus,
                                const PlainTuning* tuning = nullptr);
-int plain_blocks_per_lane(int64_t nblocks, int dim, int lanes_per_block, int num_cus);
+int plain_blocks_per_lane(int64_t nblocks, int dim, int lanes_per_block, int num_cus, bool packed = false);
 PlainTuning default_plain_tuning(const astc_image_desc& d, int64_t nblocks, int num_cus);
 
 }  // namespace astc

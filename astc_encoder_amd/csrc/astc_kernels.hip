@@ -502,10 +502,12 @@ static void launch_plain_t(const PlainArgs& a, int64_t nblocks, const PlainTunin
 
 // Blocks per lane for a plain launch: the fewest (1, 2, 4 or 8) that let every
 // workgroup be resident at once (one-lane 6x6: two workgroups of four waves per
-// CU; two-lane 6x6: three; 4x4: four), so that no second round of workgroup
-// starts (table staging, first fetch) trails the first.
-int plain_blocks_per_lane(int64_t nblocks, int dim, int lanes_per_block, int num_cus) {
-    const int64_t slots = int64_t(num_cus) * (dim == 4 ? 4 : lanes_per_block == 2 ? 3 : 2);
+// CU; two-lane or packed-texel 6x6: three; 4x4: four, packed five), so that no
+// second round of workgroup starts (table staging, first fetch) trails the first.
+int plain_blocks_per_lane(int64_t nblocks, int dim, int lanes_per_block, int num_cus, bool packed) {
+    const int per_cu = dim == 4 ? (packed ? PackedWaves<4>::value : PlainWaves<4>::value)
+                                : (lanes_per_block == 2 ? 3 : packed ? PackedWaves<6>::value : PlainWaves<6>::value);
+    const int64_t slots = int64_t(num_cus) * per_cu;
     const int64_t per_step = kPlainThreads / lanes_per_block;
     int bpl = 1;
     while (bpl < 8 && (nblocks + per_step * bpl - 1) / (per_step * bpl) > slots) bpl *= 2;
@@ -540,7 +542,9 @@ hipError_t launch_encode_plain(const DeviceTables* d_tables, const astc_image_de
     PlainTuning t = tuning ? *tuning : default_plain_tuning(d, g.nblocks, num_cus);
     (void)hipGetLastError();  // report this launch's errors only, not an earlier call's
     if (d.block_dim != 6) t.lanes_per_block = 1;
-    if (t.blocks_per_lane == 0) t.blocks_per_lane = plain_blocks_per_lane(g.nblocks, d.block_dim, t.lanes_per_block, num_cus);
+    if (t.blocks_per_lane == 0)
+        t.blocks_per_lane = plain_blocks_per_lane(g.nblocks, d.block_dim, t.lanes_per_block, num_cus,
+                                                  t.packed_texels && !(d.flags & FLAG_MAX_ACCUMULATION));
     const bool alpha = d.flags & FLAG_ALPHA;
     if (d.block_dim == 4) {
         if (alpha) launch_plain_t<4, true>(a, g.nblocks, t, stream);
