@@ -79,6 +79,22 @@ __device__ __forceinline__ bool is_short(float dot) { return dot < __uint_as_flo
 
 __device__ __forceinline__ float clamp255(float x) { return fminf(fmaxf(x, 0.0f), 255.0f); }
 
+// Issue priority of the wave (s_setprio; 0 is the default).  The encode raises
+// it to 1 for its short serial phases (power iteration; endpoints and weight
+// axis; weight quantisation and packing), whose dependent instructions leave
+// issue slots that the other wave of the SIMD then fills, and keeps 0 for the
+// passes over the texels, which have independent work for every texel.  The
+// kernels' loops raise it to 2 from a block's store to the end of the next
+// block's conversion (memory and table reads).  Measured 5% faster on 6x6
+// textures than the loops' phase alone, and 9% faster than no priorities.
+__device__ __forceinline__ void issue_priority(int p) {
+    switch (p) {
+    case 0: __builtin_amdgcn_s_setprio(0); break;
+    case 1: __builtin_amdgcn_s_setprio(1); break;
+    default: __builtin_amdgcn_s_setprio(2); break;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Fetch (ASTC_Encode.hlsl:561-580 + D3D11 Load): out-of-bounds texels are zero;
 // RGB8 expands with alpha 255 and RG8 with blue 0, alpha 255; sRGB converts the
@@ -766,7 +782,9 @@ __device__ __forceinline__ Block128 encode_block(const TX& x, const PackTables* 
         }
         float m[4][4];
         covariance_matrix<NC, N>(cov, m);
+        issue_priority(1);
         principal_axis<NC, RESCALE>(m, v);
+        issue_priority(0);
     }
     texel_pass(x);
 
@@ -783,6 +801,7 @@ __device__ __forceinline__ Block128 encode_block(const TX& x, const PackTables* 
     }
     const float lo = ext[0], hi = ext[1];
     texel_pass(x);
+    issue_priority(1);
     float e0[4], e1[4];
     uint32_t e0q[4], e1q[4];
     const bool swap = endpoints<ALPHA, NC>(v, lo, hi, mean, e0, e1, e0q, e1q, alpha);
@@ -792,6 +811,7 @@ __device__ __forceinline__ Block128 encode_block(const TX& x, const PackTables* 
     float k[4];
     weight_axis<KC>(e0, e1, swap, k);
     float proj[16];
+    issue_priority(0);
     if (DIM == 6) {
         float pt[N];
 #pragma unroll
@@ -814,8 +834,11 @@ __device__ __forceinline__ Block128 encode_block(const TX& x, const PackTables* 
         maxw = fmaxf(proj[i], maxw);
     }
     uint32_t wsh[16];  // 2 * quantised weight in the low five bits
+    issue_priority(1);
     quantize_weights<ALPHA, 16>(proj, minw, maxw, wsh);
-    return pack_block<ALPHA>(e0q, e1q, wsh, pack, swap);
+    const Block128 blk = pack_block<ALPHA>(e0q, e1q, wsh, pack, swap);
+    issue_priority(0);
+    return blk;
 }
 
 // ---------------------------------------------------------------------------

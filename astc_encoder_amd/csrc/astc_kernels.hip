@@ -88,6 +88,11 @@ struct BatchRescale { static constexpr int value = (SRGB < 0 && DIM == 6) ? 1 : 
 // loads).  With prefetch the next step's texels are loaded as soon as the
 // current ones are converted (into the same registers), so the loads are in
 // flight while the block is encoded; without, after the block is stored.
+// From the store of one block to the end of the next block's conversion the
+// wave runs at issue priority 2 (issue_priority): that phase is a chain of
+// dependent memory and LDS operations with little arithmetic, and giving it
+// precedence over the other wave's encode keeps the two waves of a SIMD out of
+// phase (measured 5% faster on 6x6 and four-channel 4x4 textures).
 // FLAT: a wave whose blocks all have a flat alpha (flat_alpha) is encoded by
 // the three-channel path (same bytes); the test costs a few percent where few
 // waves are flat, so it is a plain-kernel tuning, compiled separately.
@@ -104,6 +109,7 @@ __device__ __forceinline__ void encode_loaded(const FetchArgs& f, uint32_t* dst,
         const bool flat = kFlat && __all(flat_alpha<DIM * DIM>(cur, lut + 256));
         float x[DIM * DIM][4];
         unpack_texels<DIM, SRGB, CH, METHOD == METHOD_MAX_ACCUMULATION>(cur, f, lut, x);
+        issue_priority(0);
         // the packed words are dead once converted: the next block's loads reuse them
         if (has_next && prefetch) load_block_id<DIM, CH>(f, first + o + stride, cur);
         Block128 b;
@@ -114,10 +120,12 @@ __device__ __forceinline__ void encode_loaded(const FetchArgs& f, uint32_t* dst,
         } else {
             b = encode_block<DIM, ALPHA, CH, METHOD, RESCALE>(x, trit);
         }
+        issue_priority(2);
         store_block(dst, first + o, b);
         if (!has_next) break;
         if (!prefetch) load_block_id<DIM, CH>(f, first + o + stride, cur);
     }
+    issue_priority(0);
 }
 
 template <int DIM, bool ALPHA, int SRGB, int CH, int METHOD = METHOD_PCA, bool FLAT = false>
@@ -218,11 +226,15 @@ __device__ __forceinline__ void encode_chunk_pair(const FetchArgs& f, uint32_t* 
         const bool has_next = o + stride < count;
         float x[18][4];
         unpack_texels<6, SRGB, CH, false, 3>(cur, f, lut, x);
+        issue_priority(0);
         if (has_next && prefetch) load_half_block<CH>(f, first + o + stride, half, cur);
-        store_half_block(dst, first + o, half, encode_block_pair<ALPHA, CH, RESCALE>(x, half, trit));
+        const Block128 b = encode_block_pair<ALPHA, CH, RESCALE>(x, half, trit);
+        issue_priority(2);  // as in encode_loaded
+        store_half_block(dst, first + o, half, b);
         if (!has_next) break;
         if (!prefetch) load_half_block<CH>(f, first + o + stride, half, cur);
     }
+    issue_priority(0);
 }
 
 template <bool ALPHA, bool SRGB, int CH>
