@@ -884,7 +884,9 @@ __device__ __forceinline__ Block128 encode_block_pair(const float x[18][4], int 
     float m[4][4];
     covariance_matrix<NC, N>(cov, m);
     float v[4];
+    issue_priority(1);
     principal_axis<NC, RESCALE>(m, v);
+    issue_priority(0);
 
     float lo = 1e31f, hi = -1e31f;
 #pragma unroll
@@ -895,6 +897,7 @@ __device__ __forceinline__ Block128 encode_block_pair(const float x[18][4], int 
     }
     lo = fminf(lo, pair_other(lo));
     hi = fmaxf(hi, pair_other(hi));
+    issue_priority(1);
     float e0[4], e1[4];
     uint32_t e0q[4], e1q[4];
     const bool swap = endpoints<ALPHA, NC>(v, lo, hi, mean, e0, e1, e0q, e1q);
@@ -902,6 +905,7 @@ __device__ __forceinline__ Block128 encode_block_pair(const float x[18][4], int 
     constexpr int KC = (NC == 4 && !ALPHA) ? 3 : NC;
     float k[4];
     weight_axis<KC>(e0, e1, swap, k);
+    issue_priority(0);
     float pt[NH];
 #pragma unroll
     for (int j = 0; j < NH; ++j) pt[j] = dotn<KC>(k, x[j]);
@@ -916,6 +920,7 @@ __device__ __forceinline__ Block128 encode_block_pair(const float x[18][4], int 
     minw = fminf(minw, pair_other(minw));
     maxw = fmaxf(maxw, pair_other(maxw));
     uint32_t own[8];
+    issue_priority(1);
     quantize_weights<ALPHA, 8>(proj, minw, maxw, own);
     uint32_t packed = 0;  // the eight quantised weights, 4 bits each
 #pragma unroll
@@ -928,7 +933,9 @@ __device__ __forceinline__ Block128 encode_block_pair(const float x[18][4], int 
         wsh[i] = ((first >> (4 * i)) & 15u) << 1;
         wsh[8 + i] = ((second >> (4 * i)) & 15u) << 1;
     }
-    return pack_block<ALPHA>(e0q, e1q, wsh, pack, swap);
+    const Block128 blk = pack_block<ALPHA>(e0q, e1q, wsh, pack, swap);
+    issue_priority(0);
+    return blk;
 }
 
 }  // namespace astc

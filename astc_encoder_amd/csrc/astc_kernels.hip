@@ -305,9 +305,11 @@ __device__ __forceinline__ void process_item(const BatchArgs& a, const BatchItem
         if (o < sg.count) {
             const BatchLevel& L = a.levels[sg.level];
             uint32_t p[Layout<DIM, CH>::WORDS];
+            issue_priority(2);  // fetch and conversion, as in encode_loaded
             load_block_id<DIM, CH>(L.f, sg.first_block + o, p);
             float x[DIM * DIM][4];
             unpack_texels<DIM, -1, CH>(p, L.f, lut, x);
+            issue_priority(0);
             store_block(L.dst, sg.first_block + o, encode_block<DIM, ALPHA, CH, METHOD_PCA, BatchRescale<DIM, -1>::value>(x, trit));
             return;
         }
