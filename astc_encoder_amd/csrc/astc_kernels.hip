@@ -87,11 +87,11 @@ struct BatchRescale { static constexpr int value = (SRGB < 0 && DIM == 6) ? 1 : 
 // loads).  With prefetch the next step's texels are loaded as soon as the
 // current ones are converted (into the same registers), so the loads are in
 // flight while the block is encoded; without, after the block is stored.
-// From the store of one block to the end of the next block's conversion the
-// wave runs at issue priority 2 (issue_priority): that phase is a chain of
-// dependent memory and LDS operations with little arithmetic, and giving it
-// precedence over the other wave's encode keeps the two waves of a SIMD out of
-// phase (measured 5% faster on 6x6 and four-channel 4x4 textures).
+// From the store of one block to the end of the next block's conversion (and,
+// with prefetch, the issue of the following block's loads) the wave runs at
+// issue priority 2 (issue_priority): that phase is a chain of dependent memory
+// and LDS operations with little arithmetic, and giving it precedence over the
+// other wave's encode keeps the two waves of a SIMD out of phase.
 // FLAT: a wave whose blocks all have a flat alpha (flat_alpha) is encoded by
 // the three-channel path (same bytes); the test costs a few percent where few
 // waves are flat, so it is a plain-kernel tuning, compiled separately.
@@ -108,9 +108,9 @@ __device__ __forceinline__ void encode_loaded(const FetchArgs& f, uint32_t* dst,
         const bool flat = kFlat && __all(flat_alpha<DIM * DIM>(cur, lut + 256));
         float x[DIM * DIM][4];
         unpack_texels<DIM, SRGB, CH, METHOD == METHOD_MAX_ACCUMULATION>(cur, f, lut, x);
-        issue_priority(0);
         // the packed words are dead once converted: the next block's loads reuse them
         if (has_next && prefetch) load_block_id<DIM, CH>(f, first + o + stride, cur);
+        issue_priority(0);
         Block128 b;
         if (kFlat && flat) {
             // keeps the two encoders' code apart: merged, their live values exceed the register file
@@ -132,6 +132,7 @@ __device__ __forceinline__ void encode_chunk(const FetchArgs& f, uint32_t* dst, 
                                              const float* lut, const PackTables* trit) {
     if (int(threadIdx.x) >= count) return;
     uint32_t cur[Layout<DIM, CH>::WORDS];
+    issue_priority(2);  // the first block's fetch, like every later one (encode_loaded)
     load_block_id<DIM, CH>(f, first + threadIdx.x, cur);
     encode_loaded<DIM, ALPHA, SRGB, CH, METHOD, FLAT>(f, dst, first, count, prefetch, lut, trit, cur);
 }
@@ -518,6 +519,11 @@ static void launch_plain_t(const PlainArgs& a, int64_t nblocks, const PlainTunin
 // CU; two-lane or packed-texel 6x6: three; 4x4: four, packed five), so that no
 // second round of workgroup starts (table staging, first fetch) trails the first.
 int plain_blocks_per_lane(int64_t nblocks, int dim, int lanes_per_block, int num_cus, bool packed) {
+    // 4x4 one-lane encodes: one block per lane, whatever the count; the
+    // workgroups that do not fit at first start as others finish, which
+    // balances the end of the launch (4096x4096 RGBA8: 21.4 us against 23.0 us
+    // at the resident-workgroup rule below)
+    if (dim == 4 && !packed) return 1;
     const int per_cu = dim == 4 ? (packed ? PackedWaves<4>::value : PlainWaves<4>::value)
                                 : (lanes_per_block == 2 ? 3 : packed ? PackedWaves<6>::value : PlainWaves<6>::value);
     const int64_t slots = int64_t(num_cus) * per_cu;
