@@ -93,6 +93,7 @@ SIGNATURES = {
     "astc_encode": (I32, [P, ctypes.POINTER(ImageDesc), P, P, P]),
     "astc_encode_tuned": (I32, [P, ctypes.POINTER(ImageDesc), P, P, ctypes.POINTER(PlainTuning), P]),
     "astc_plain_autotune": (I32, [P, ctypes.POINTER(ImageDesc), P, P, P, I32, ctypes.POINTER(PlainTuning)]),
+    "astc_plain_default_tuning": (I32, [P, ctypes.POINTER(ImageDesc), ctypes.POINTER(PlainTuning)]),
     "astc_decode": (I32, [P, I32, I32, I32, I32, P, P, P]),
     "astc_encode_each": (I32, [P, PDESC, ctypes.POINTER(P), ctypes.POINTER(P), I32, P]),
     "astc_write_file": (I32, [ctypes.c_char_p, I32, I32, I32, I32, P, I64]),

@@ -201,6 +201,22 @@ int astc_plain_autotune(astc_ctx* ctx, const astc_image_desc* desc, const void* 
     });
 }
 
+int astc_plain_default_tuning(astc_ctx* ctx, const astc_image_desc* desc, astc_plain_tuning* tuning) {
+    return guarded([&]() -> int {
+        if (!ctx || !desc || !tuning) return fail(ASTC_ERR_INVALID_ARG, "NULL argument");
+        ImageGeom g;
+        const int rc = check_desc(desc, &g);
+        if (rc != ASTC_OK) return rc;
+        const PlainTuning t = default_plain_tuning(*desc, g.nblocks, ctx->num_cus);
+        tuning->lanes_per_block = t.lanes_per_block;
+        tuning->blocks_per_lane = t.blocks_per_lane;
+        tuning->prefetch = t.prefetch;
+        tuning->flat_alpha = t.flat_alpha;
+        tuning->packed_texels = t.packed_texels;
+        return ASTC_OK;
+    });
+}
+
 int astc_decode(astc_ctx* ctx, int width, int height, int block_dim_x, int block_dim_y, const void* blocks_dev,
                 void* rgba_dev, void* stream) {
     return guarded([&]() -> int {

@@ -168,6 +168,13 @@ class Encoder:
         return t.as_dict()
 
 
+    def default_plain_tuning(self, texture, block_dim=4, flags=0):
+        """astc_plain_default_tuning: the tuning encode() uses for this texture, as a dict."""
+        desc = self._texture_desc(texture, block_dim, flags)
+        t = PlainTuning()
+        check(lib().astc_plain_default_tuning(self._ctx, ctypes.byref(desc), ctypes.byref(t)))
+        return t.as_dict()
+
     def decode(self, blocks, width, height, block_dim_x=4, block_dim_y=None, out=None, stream=None):
         """Decode (nblocks, 16) device blocks to an (H, W, 4) uint8 device tensor."""
         import torch

@@ -85,11 +85,16 @@ def time_plain(enc, wl, steps, warmup, tuning=None):
                           wl["dim"], wl["flags"])
     ctx, dref = enc.handle, ctypes.byref(desc)
     sp, dp, st = ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream.cuda_stream)
+    from astc_encoder_amd._capi import PlainTuning
     if tuning is None:
         fn, args = astc.lib().astc_encode, (ctx, dref, sp, dp, st)
+        used = PlainTuning()
+        if astc.lib().astc_plain_default_tuning(ctx, dref, ctypes.byref(used)) != 0:
+            raise astc.AstcError(-2, astc.lib().astc_last_error().decode())
+        tuning_used = (used.lanes_per_block, used.blocks_per_lane, used.prefetch, used.flat_alpha, used.packed_texels)
     else:
-        from astc_encoder_amd._capi import PlainTuning
-        t = PlainTuning(*(tuple(tuning) + (0,) * (5 - len(tuning))))
+        tuning_used = tuple(tuning) + (0,) * (5 - len(tuning))
+        t = PlainTuning(*tuning_used)
         fn, args = astc.lib().astc_encode_tuned, (ctx, dref, sp, dp, ctypes.byref(t), st)
 
     def launch():
@@ -124,7 +129,7 @@ def time_plain(enc, wl, steps, warmup, tuning=None):
     texels = wl["w"] * wl["h"]
     nblocks = out.shape[0]
     return dict(img=img, blocks=out.cpu().numpy(), wall_s=wall, kernel_ms=kernel_ms, step_event_ms=step_event_ms,
-                texels=texels,
+                texels=texels, tuning=tuning_used,
                 nblocks=nblocks, mtexels_s=texels * steps / wall / 1e6, psnr_db=gpu_psnr(enc, wl, src, out))
 
 
@@ -490,14 +495,18 @@ def main():
             "flops_per_block": FLOPS_PER_BLOCK[head["flops_key"]]}
     # the roofline that binds is the one the kernel comes closest to
     bound = dict(hbm if hbm["frac"] >= valu["frac"] else valu)
-    if head_tuning and head_tuning[0] == 2:
+    # the kernel the launches ran: from the tuning they used (the library's default
+    # for the image when none is given, as astc_plain_default_tuning reports it)
+    used = r["tuning"]
+    if used[0] == 2:
         kernel = "encode_plain_pair_kernel<true, true, 4>"
-    elif head_tuning and len(head_tuning) > 4 and head_tuning[4]:
+    elif used[4]:
         kernel = "encode_plain_packed_kernel<6, true, true, 4>"
     else:
-        flat = bool(head_tuning and len(head_tuning) > 3 and head_tuning[3])
-        kernel = f"encode_plain_kernel<6, true, true, 4, 0, {'true' if flat else 'false'}>"
-    bound.update({"kernel": kernel, "plain_tuning": head_tuning or "library default",
+        kernel = f"encode_plain_kernel<6, true, true, 4, 0, {'true' if used[3] else 'false'}>"
+    tuning_fields = ("lanes_per_block", "blocks_per_lane", "prefetch", "flat_alpha", "packed_texels")
+    bound.update({"kernel": kernel,
+                  "plain_tuning": dict(zip(tuning_fields, used), source="given" if head_tuning else "library default"),
                   "kernel_ms": round(r["kernel_ms"], 4),
                   "step_event_ms": round(r["step_event_ms"], 4), "algorithmic_bytes": algo_bytes,
                   "algorithmic_flops": flops})

@@ -127,6 +127,10 @@ int astc_encode_tuned(astc_ctx* ctx, const astc_image_desc* desc, const void* sr
  * *best.  No reference counterpart (the reference has one fixed dispatch). */
 int astc_plain_autotune(astc_ctx* ctx, const astc_image_desc* desc, const void* src_dev, void* dst_dev, void* stream,
                         int reps, astc_plain_tuning* best);
+/* The tuning astc_encode uses for this image (blocks_per_lane resolved; fields
+ * that do not apply to the image's channel class or method are 0).  No
+ * reference counterpart. */
+int astc_plain_default_tuning(astc_ctx* ctx, const astc_image_desc* desc, astc_plain_tuning* tuning);
 
 /* Decode blocks (row-major, ceil(W/bx) x ceil(H/by) of them, 16-byte aligned)
  * to a width x height RGBA8 image (4-byte aligned, row pitch width * 4).

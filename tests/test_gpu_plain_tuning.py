@@ -155,3 +155,26 @@ def test_packed_tuning_with_max_accumulation_runs_the_default_kernel(enc, dim):
     flags = astc.FLAG_ALPHA | astc.FLAG_MAX_ACCUMULATION
     want = orc.oracle_encode(img, dim, flags)
     assert_same(tuned(enc, torch.from_numpy(img).cuda(), dim, flags, (1, 0, 0, 0, 1)), want, f"max-acc {dim}")
+
+
+@pytest.mark.parametrize("channels,dim,flags,shape,want", [
+    (4, 6, astc.FLAG_ALPHA | astc.FLAG_SRGB, (4096, 4096), dict(prefetch=1, flat_alpha=1, blocks_per_lane=4)),
+    (4, 4, astc.FLAG_ALPHA, (1024, 1024), dict(prefetch=1, flat_alpha=1, blocks_per_lane=1)),
+    (3, 6, 0, (96, 96), dict(prefetch=0, flat_alpha=0)),          # tiled RGB8
+    (3, 6, 0, (97, 96), dict(prefetch=1, flat_alpha=1)),          # ragged RGB8: four-channel path
+    (2, 4, astc.FLAG_NORMAL, (256, 256), dict(prefetch=1, flat_alpha=0)),
+    (4, 6, astc.FLAG_ALPHA | astc.FLAG_MAX_ACCUMULATION, (128, 128), dict(prefetch=1, flat_alpha=0)),
+])
+def test_default_tuning_is_reported_and_matches_astc_encode(enc, channels, dim, flags, shape, want):
+    rng = np.random.default_rng(channels * 10 + dim)
+    img = rng.integers(0, 256, size=(shape[1], shape[0], channels), dtype=np.uint8)
+    tex = torch.from_numpy(img).cuda()
+    t = enc.default_plain_tuning(tex, dim, flags)
+    assert t["lanes_per_block"] == 1 and t["packed_texels"] == 0
+    for k, v in want.items():
+        assert t[k] == v, (k, t)
+    default = enc.encode(tex, block_dim=dim, flags=flags)
+    tuned = enc.encode(tex, block_dim=dim, flags=flags,
+                       tuning=(t["lanes_per_block"], t["blocks_per_lane"], t["prefetch"], t["flat_alpha"], 0))
+    torch.cuda.synchronize()
+    assert torch.equal(default, tuned)
