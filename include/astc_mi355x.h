@@ -103,8 +103,9 @@ int astc_encode(astc_ctx* ctx, const astc_image_desc* desc, const void* src_dev,
  * before the current block is encoded.  flat_alpha: 1 tests each wave of
  * four-channel blocks (RGBA8 sources, one lane per block) for an alpha that is
  * the same integer-valued byte in every texel of every block, and encodes such
- * waves with the three-channel encoder (faster on opaque textures, a little
- * slower where few waves qualify).  packed_texels: 1 keeps each block's texels
+ * waves with the three-channel encoder (faster on opaque textures, about
+ * neutral where few waves qualify; astc_encode uses it for four-channel
+ * encodes).  packed_texels: 1 keeps each block's texels
  * as the fetched bytes and converts them again in every pass of the encode
  * (fewer registers, more waves per SIMD, more instructions); it needs
  * lanes_per_block 1, prefetch 0 and flat_alpha 0, and applies to the principal
