@@ -95,7 +95,7 @@ struct BatchRescale { static constexpr int value = (SRGB < 0 && DIM == 6) ? 1 : 
 // FLAT: a wave whose blocks all have a flat alpha (flat_alpha) is encoded by
 // the three-channel path (same bytes); a separately compiled kernel, which the
 // default plain tuning uses for four-channel encodes (the batch kernels do not
-// test: their register budget has no room for both encoders).
+// test for flat alpha).
 template <int DIM, bool ALPHA, int SRGB, int CH, int METHOD = METHOD_PCA, bool FLAT = false>
 __device__ __forceinline__ void encode_loaded(const FetchArgs& f, uint32_t* dst, int first, int count, bool prefetch,
                                               const float* lut, const PackTables* trit,
