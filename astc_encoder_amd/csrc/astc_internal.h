@@ -173,6 +173,8 @@ static_assert(sizeof(NoAllocState::keys) / sizeof(uint64_t) == 2 * kNoAllocMaxIm
 int noalloc_setup(astc_ctx* ctx);
 void noalloc_teardown(astc_ctx* ctx);
 
+// Device-to-device copy of `bytes` on `stream` with the library's copy kernel.
+hipError_t launch_ring_copy(void* dst, const void* src, int64_t bytes, hipStream_t stream);
 hipError_t launch_downsample(const uint8_t* src, int w, int h, int bpp, uint8_t* dst, int ow, int oh,
                              hipStream_t stream);
 

@@ -448,6 +448,32 @@ __global__ void __launch_bounds__(256) downsample_kernel(const uint8_t* __restri
     dst[i] = uint8_t(v);
 }
 
+// Device-to-device copy for the server's ring (sources in, blocks out): 16-byte
+// vectors when both ends and the size allow, bytes otherwise; grid-stride.  A
+// kernel of the library's own keeps these copies on the compute queue of the
+// slot's stream.
+__global__ void __launch_bounds__(256) ring_copy_kernel(uint8_t* __restrict__ dst, const uint8_t* __restrict__ src,
+                                                        int64_t bytes) {
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src) | uintptr_t(bytes)) & 15u) == 0) {
+        const int64_t n = bytes >> 4;
+        for (int64_t i = t; i < n; i += stride)
+            reinterpret_cast<uint4*>(dst)[i] = reinterpret_cast<const uint4*>(src)[i];
+    } else {
+        for (int64_t i = t; i < bytes; i += stride) dst[i] = src[i];
+    }
+}
+
+hipError_t launch_ring_copy(void* dst, const void* src, int64_t bytes, hipStream_t stream) {
+    if (bytes <= 0) return hipSuccess;
+    const int64_t per_block = 256 * 16 * 4;  // four vectors per thread per pass
+    const unsigned blocks = unsigned(std::min<int64_t>(2048, (bytes + per_block - 1) / per_block));
+    hipLaunchKernelGGL(ring_copy_kernel, dim3(blocks), dim3(256), 0, stream, static_cast<uint8_t*>(dst),
+                       static_cast<const uint8_t*>(src), bytes);
+    return hipGetLastError();
+}
+
 hipError_t launch_downsample(const uint8_t* src, int w, int h, int bpp, uint8_t* dst, int ow, int oh,
                              hipStream_t stream) {
     const int64_t total = int64_t(ow) * oh * bpp;

@@ -101,6 +101,14 @@ struct Slot {
     BatchSeg* h_segs = nullptr;
     BatchItem* h_items = nullptr;
     int* h_scalars = nullptr;
+    // device addresses of the pinned staging: the work list is fetched into the
+    // device buffers by the library's copy kernel on the slot's stream (the
+    // runtime's DMA path for these uploads stalled the serving loop for 5-9 ms
+    // now and then)
+    const void* dv_levels = nullptr;
+    const void* dv_segs = nullptr;
+    const void* dv_items = nullptr;
+    const void* dv_scalars = nullptr;
     size_t cap_levels = 0, cap_segs = 0, cap_items = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -181,6 +189,7 @@ void free_slot_buffers(Slot& s) {
     for (void* p : {(void*)s.h_levels, (void*)s.h_segs, (void*)s.h_items}) if (p) (void)hipHostFree(p);
     s.d_levels = nullptr; s.d_segs = nullptr; s.d_items = nullptr;
     s.h_levels = nullptr; s.h_segs = nullptr; s.h_items = nullptr;
+    s.dv_levels = nullptr; s.dv_segs = nullptr; s.dv_items = nullptr;
     s.cap_levels = s.cap_segs = s.cap_items = 0;
 }
 
@@ -200,6 +209,10 @@ int ensure_capacity(astc_server* sv, Slot& s, size_t nlev, size_t nseg, size_t n
         (e = hipHostMalloc(&s.h_segs, s.cap_segs * sizeof(BatchSeg), 0)) != hipSuccess ||
         (e = hipHostMalloc(&s.h_items, s.cap_items * sizeof(BatchItem), 0)) != hipSuccess)
         return hip_fail(e, "hipHostMalloc(staging)");
+    if ((e = hipHostGetDevicePointer(const_cast<void**>(&s.dv_levels), s.h_levels, 0)) != hipSuccess ||
+        (e = hipHostGetDevicePointer(const_cast<void**>(&s.dv_segs), s.h_segs, 0)) != hipSuccess ||
+        (e = hipHostGetDevicePointer(const_cast<void**>(&s.dv_items), s.h_items, 0)) != hipSuccess)
+        return hip_fail(e, "hipHostGetDevicePointer(staging)");
     BatchArgs a;
     a.levels = s.d_levels;
     a.segs = s.d_segs;
@@ -318,16 +331,12 @@ int launch_batch(astc_server* sv, Slot& s) {
             last_wait = en.ready;
         }
         if (e == hipSuccess && en.copy_src)
-            e = hipMemcpyAsync(sv->ring_base + en.off, en.copy_src, en.geo->src_copy, hipMemcpyDeviceToDevice, s.stream);
+            e = launch_ring_copy(sv->ring_base + en.off, en.copy_src, en.geo->src_copy, s.stream);
     }
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(s.d_levels, s.h_levels, levels.size() * sizeof(BatchLevel), hipMemcpyHostToDevice, s.stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(s.d_segs, s.h_segs, sv->segs.size() * sizeof(BatchSeg), hipMemcpyHostToDevice, s.stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(s.d_items, s.h_items, sv->items.size() * sizeof(BatchItem), hipMemcpyHostToDevice,
-                           s.stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(s.d_scalars, s.h_scalars, 4 * sizeof(int), hipMemcpyHostToDevice, s.stream);
+    if (e == hipSuccess) e = launch_ring_copy(s.d_levels, s.dv_levels, levels.size() * sizeof(BatchLevel), s.stream);
+    if (e == hipSuccess) e = launch_ring_copy(s.d_segs, s.dv_segs, sv->segs.size() * sizeof(BatchSeg), s.stream);
+    if (e == hipSuccess) e = launch_ring_copy(s.d_items, s.dv_items, sv->items.size() * sizeof(BatchItem), s.stream);
+    if (e == hipSuccess) e = launch_ring_copy(s.d_scalars, s.dv_scalars, 4 * sizeof(int), s.stream);
     if (e == hipSuccess) e = hipGraphLaunch(s.exec, s.stream);
     for (const BatchEntry& en : sv->entries) {
         const ReqGeom& q = *en.geo;
@@ -341,8 +350,7 @@ int launch_batch(astc_server* sv, Slot& s) {
     }
     for (const BatchEntry& en : sv->entries)
         if (e == hipSuccess && en.copy_dst)
-            e = hipMemcpyAsync(en.copy_dst, sv->ring_base + en.off + en.geo->src_bytes, en.geo->out_bytes,
-                               hipMemcpyDeviceToDevice, s.stream);
+            e = launch_ring_copy(en.copy_dst, sv->ring_base + en.off + en.geo->src_bytes, en.geo->out_bytes, s.stream);
     if (e == hipSuccess) e = hipEventRecord(s.done, s.stream);
     if (e != hipSuccess) return hip_fail(e, "batch submission");
     s.busy = true;
@@ -409,7 +417,8 @@ int astc_server_create(astc_ctx* ctx, const astc_server_config* cfg, astc_server
                     (e = hipEventCreateWithFlags(&s.fork, hipEventDisableTiming)) != hipSuccess ||
                     (e = hipEventCreateWithFlags(&s.join, hipEventDisableTiming)) != hipSuccess ||
                     (e = hipMalloc(&s.d_scalars, 4 * sizeof(int))) != hipSuccess ||
-                    (e = hipHostMalloc(&s.h_scalars, 4 * sizeof(int), 0)) != hipSuccess) {
+                    (e = hipHostMalloc(&s.h_scalars, 4 * sizeof(int), 0)) != hipSuccess ||
+                    (e = hipHostGetDevicePointer(const_cast<void**>(&s.dv_scalars), s.h_scalars, 0)) != hipSuccess) {
                     rc = hip_fail(e, "slot setup");
                     break;
                 }
@@ -423,17 +432,15 @@ int astc_server_create(astc_ctx* ctx, const astc_server_config* cfg, astc_server
             }
         }
         // Replay every slot's graph once on an empty work list (the kernels exit at
-        // once), with a ring-to-ring copy and an event on the slot's stream, so that
-        // the runtime's first-use work (queue assignment, code-object and copy-kernel
-        // loading) happens here rather than under the first requests.
+        // once), with the work-list fetch, a ring-to-ring copy and an event on the
+        // slot's stream, so that the runtime's first-use work (queue assignment,
+        // code-object loading) happens here rather than under the first requests.
         for (Slot& s : sv->slots) {
             if (rc != ASTC_OK) break;
             s.h_scalars[0] = s.h_scalars[1] = s.h_scalars[2] = s.h_scalars[3] = 0;
-            if ((e = hipMemcpyAsync(s.d_scalars, s.h_scalars, 4 * sizeof(int), hipMemcpyHostToDevice, s.stream)) !=
-                    hipSuccess ||
+            if ((e = launch_ring_copy(s.d_scalars, s.dv_scalars, 4 * sizeof(int), s.stream)) != hipSuccess ||
                 (e = hipGraphLaunch(s.exec, s.stream)) != hipSuccess ||
-                (e = hipMemcpyAsync(sv->ring_base + 128, sv->ring_base, 128, hipMemcpyDeviceToDevice, s.stream)) !=
-                    hipSuccess ||
+                (e = launch_ring_copy(sv->ring_base + 128, sv->ring_base, 128, s.stream)) != hipSuccess ||
                 (e = hipEventRecord(s.done, s.stream)) != hipSuccess || (e = hipEventSynchronize(s.done)) != hipSuccess)
                 rc = hip_fail(e, "slot warm-up");
         }
