@@ -22,6 +22,7 @@ struct NoAllocPlan;
 struct NoAllocState {
     void* scratch = nullptr;
     NoAllocPlan* staging = nullptr;
+    const void* staging_dev = nullptr;  // the staging's device address (read by the copy kernel)
     hipEvent_t staged = nullptr;
     std::mutex mu;
     uint64_t keys[2 * 8192];

@@ -11,8 +11,9 @@
 // device is taken once, in astc_ctx_create (noalloc_setup, with the device
 // current), and only that address is used on the host.  The plan is written
 // into pinned host staging that the context allocated at creation, copied into
-// scratch on the caller's stream, and the staging is reused only after that
-// copy has completed (an event).  The queue heads are cleared by a kernel on
+// scratch on the caller's stream by the library's copy kernel (which reads the
+// staging through its device address), and the staging is reused only after
+// that copy has completed (an event).  The queue heads are cleared by a kernel on
 // the same stream before the persistent kernel runs.
 //
 // Every call on one GPU uses that GPU's scratch: calls on one stream follow
@@ -34,6 +35,11 @@ int noalloc_setup(astc_ctx* ctx) {
         ctx->noalloc.staging = nullptr;
         return hip_fail(e, "hipHostMalloc(no-alloc staging)");
     }
+    if ((e = hipHostGetDevicePointer(const_cast<void**>(&ctx->noalloc.staging_dev), ctx->noalloc.staging, 0)) !=
+        hipSuccess) {
+        ctx->noalloc.staging_dev = nullptr;
+        return hip_fail(e, "hipHostGetDevicePointer(no-alloc staging)");
+    }
     if ((e = hipEventCreateWithFlags(&ctx->noalloc.staged, hipEventDisableTiming)) != hipSuccess) {
         ctx->noalloc.staged = nullptr;
         return hip_fail(e, "hipEventCreate(no-alloc staging)");
@@ -49,6 +55,7 @@ void noalloc_teardown(astc_ctx* ctx) {
     if (ctx->noalloc.staging) (void)hipHostFree(ctx->noalloc.staging);
     ctx->noalloc.staged = nullptr;
     ctx->noalloc.staging = nullptr;
+    ctx->noalloc.staging_dev = nullptr;
     ctx->noalloc.scratch = nullptr;
 }
 
@@ -60,7 +67,7 @@ extern "C" int astc_encode_batch_noalloc(astc_ctx* ctx, const astc_image_desc* d
                                          void* const* dsts, int count, const astc_tuning* tuning, void* stream) {
     return guarded([&]() -> int {
         if (!ctx || !descs || !srcs || !dsts || count < 1) return fail(ASTC_ERR_INVALID_ARG, "empty or NULL batch");
-        if (!ctx->noalloc.scratch || !ctx->noalloc.staging || !ctx->noalloc.staged)
+        if (!ctx->noalloc.scratch || !ctx->noalloc.staging || !ctx->noalloc.staging_dev || !ctx->noalloc.staged)
             return fail(ASTC_ERR_INVALID_ARG, "context has no no-alloc scratch");
         if (count > kNoAllocMaxImages) return fail(ASTC_ERR_INVALID_ARG, "more images than the no-alloc maximum (8192)");
         const astc_tuning t = tuning ? *tuning : astc_tuning{4, 2, 1, 0, 1};
@@ -124,10 +131,12 @@ extern "C" int astc_encode_batch_noalloc(astc_ctx* ctx, const astc_image_desc* d
             {offsetof(NoAllocPlan, segs), size_t(nseg) * sizeof(BatchSeg)},
             {offsetof(NoAllocPlan, items), size_t(nitem) * sizeof(BatchItem)},
             {offsetof(NoAllocPlan, scalars), sizeof(P.scalars)}};
+        // fetched by the library's copy kernel through the staging's device address
+        // (no DMA-engine copy: those stalled the host for milliseconds now and then)
+        const uint8_t* staged_dev = static_cast<const uint8_t*>(ctx->noalloc.staging_dev);
         for (const auto& part : parts) {
             if (!part.bytes) continue;
-            if ((e = hipMemcpyAsync(base + part.off, reinterpret_cast<const uint8_t*>(&P) + part.off, part.bytes,
-                                    hipMemcpyHostToDevice, s)) != hipSuccess)
+            if ((e = launch_ring_copy(base + part.off, staged_dev + part.off, int64_t(part.bytes), s)) != hipSuccess)
                 return hip_fail(e, "no-alloc work-list upload");
         }
         if ((e = hipEventRecord(ctx->noalloc.staged, s)) != hipSuccess) return hip_fail(e, "no-alloc staging event");

@@ -214,7 +214,8 @@ int astc_batch_destroy(astc_batch* batch);
 /* Encode `count` images with the batched persistent kernel in one call that
  * allocates no device or host memory: the work list and queue heads live in a
  * fixed array in the library (4.5 MiB per GPU, held until the process ends),
- * uploaded from pinned staging the context allocated at creation.  No
+ * fetched by a copy kernel from pinned staging the context allocated at
+ * creation.  No
  * counterpart in the reference: this is for callers that cannot allocate
  * (code under graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  Images and outputs as for astc_batch_create; each image gets
